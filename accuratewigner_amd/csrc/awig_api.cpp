@@ -1,0 +1,330 @@
+// libawig C ABI: plans, fused passes, factor builders, reductions and hipRTC modules.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "awig_internal.h"
+
+using namespace awig;
+
+struct aw_plan {
+    int rows, cols, batch, prec;
+    int csize;                 // bytes per complex element
+    void *tw_rows = nullptr;   // roots for FFTs of length rows (axis 0)
+    void *tw_cols = nullptr;   // roots for FFTs of length cols (axis 1)
+    double *partials = nullptr;
+    int red_blocks = 0;
+};
+
+struct aw_module {
+    hipModule_t mod = nullptr;
+    std::string name;
+};
+
+#define AW_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) return set_error(AW_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// exp(-2 pi i t / n) in FP64 with exact octant reduction of t/n before calling libm
+static void host_root(long long t, long long n, double &c, double &s) {
+    long long a = ((t % n) + n) % n;
+    bool neg_s = false, neg_c = false, swap = false;
+    if (2 * a > n) { a = n - a; neg_s = true; }
+    if (4 * a > n && n % 2 == 0) { a = n / 2 - a; neg_c = true; }
+    if (8 * a > n && n % 8 == 0) { a = n / 4 - a; swap = true; }
+    const double th = 2.0 * 3.141592653589793238462643383279502884 * double(a) / double(n);
+    double cc = std::cos(th), ss = std::sin(th);
+    if (swap) std::swap(cc, ss);
+    if (neg_c) cc = -cc;
+    if (neg_s) ss = -ss;
+    c = cc;
+    s = -ss;  // forward root exp(-i theta)
+}
+
+static int make_twiddles(int n, int prec, void **out) {
+    std::vector<double> h(2 * (size_t)n);
+    for (int t = 0; t < n; ++t) host_root(t, n, h[2 * t], h[2 * t + 1]);
+    size_t bytes;
+    std::vector<float> hf;
+    const void *src = h.data();
+    if (prec == AW_PREC_F32) {
+        hf.resize(2 * (size_t)n);
+        for (size_t i = 0; i < hf.size(); ++i) hf[i] = (float)h[i];
+        src = hf.data();
+        bytes = hf.size() * sizeof(float);
+    } else {
+        bytes = h.size() * sizeof(double);
+    }
+    AW_HIP(hipMalloc(out, bytes));
+    AW_HIP(hipMemcpy(*out, src, bytes, hipMemcpyHostToDevice));
+    return AW_OK;
+}
+
+extern "C" {
+
+int aw_version(void) { return 100; }
+
+const char *aw_last_error(void) { return g_err.c_str(); }
+
+int aw_plan_create(aw_plan **plan, int rows, int cols, int batch, int prec) {
+    if (!plan) return set_error(AW_ERR_INVALID_VALUE, "plan pointer is NULL");
+    *plan = nullptr;
+    if (prec != AW_PREC_F64 && prec != AW_PREC_F32)
+        return set_error(AW_ERR_INVALID_TYPE, "unsupported precision %d", prec);
+    if (batch < 1) return set_error(AW_ERR_INVALID_SIZE, "batch must be >= 1 (got %d)", batch);
+    if (!is_pow2(cols) || cols < 64 || cols > 4096)
+        return set_error(AW_ERR_INVALID_SIZE, "cols = %d: must be a power of two in 64..4096", cols);
+    if (!(rows == 1 || (is_pow2(rows) && rows >= 64 && rows <= 4096)))
+        return set_error(AW_ERR_INVALID_SIZE, "rows = %d: must be 1 or a power of two in 64..4096", rows);
+    aw_plan *p = new aw_plan();
+    p->rows = rows;
+    p->cols = cols;
+    p->batch = batch;
+    p->prec = prec;
+    p->csize = prec == AW_PREC_F64 ? 16 : 8;
+    int rc = make_twiddles(cols, prec, &p->tw_cols);
+    if (rc == AW_OK && rows > 1) rc = make_twiddles(rows, prec, &p->tw_rows);
+    if (rc == AW_OK) {
+        const long long n = (long long)rows * cols;
+        long long nb = (n + 256 * 8 - 1) / (256 * 8);
+        if (nb > 1024) nb = 1024;
+        if (nb < 1) nb = 1;
+        p->red_blocks = (int)nb;
+        hipError_t e = hipMalloc((void **)&p->partials, sizeof(double) * 8 * (size_t)nb * batch);
+        if (e != hipSuccess) rc = set_error(AW_ERR_ALLOC_FAILED, "partials: %s", hipGetErrorString(e));
+    }
+    if (rc != AW_OK) {
+        aw_plan_destroy(p);
+        return rc;
+    }
+    *plan = p;
+    return AW_OK;
+}
+
+int aw_plan_destroy(aw_plan *p) {
+    if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
+    if (p->tw_rows) (void)hipFree(p->tw_rows);
+    if (p->tw_cols) (void)hipFree(p->tw_cols);
+    if (p->partials) (void)hipFree(p->partials);
+    delete p;
+    return AW_OK;
+}
+
+int aw_pass_run(aw_plan *p, void *data, const aw_pass *ps, aw_stream stream) {
+    if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
+    if (!data || !ps) return set_error(AW_ERR_INVALID_VALUE, "NULL data or pass");
+    if (ps->axis != 0 && ps->axis != 1) return set_error(AW_ERR_INVALID_VALUE, "axis must be 0 or 1");
+    if (ps->axis == 0 && p->rows == 1)
+        return set_error(AW_ERR_INVALID_VALUE, "axis-0 pass on a single-row plan");
+    for (int d : {ps->dir_a, ps->dir_b})
+        if (d != 0 && d != AW_FORWARD && d != AW_INVERSE)
+            return set_error(AW_ERR_INVALID_VALUE, "FFT direction must be 0, -1 or +1 (got %d)", d);
+    if (ps->out_mode == 1 && p->rows != p->cols)
+        return set_error(AW_ERR_INVALID_VALUE, "diagonal output needs a square grid");
+    PassArgs a{};
+    a.in = data;
+    a.out = ps->out ? ps->out : data;
+    a.bstride = (long long)p->rows * p->cols;
+    const bool strided = ps->axis == 0;
+    const int n = strided ? p->rows : p->cols;
+    a.nlines = strided ? p->cols : p->rows;
+    a.line_stride = strided ? 1 : p->cols;
+    a.elem_stride = strided ? p->cols : 1;
+    a.total_lines = (long long)a.nlines * p->batch;
+    a.dir_a = ps->dir_a;
+    a.dir_b = ps->dir_b;
+    a.sign_in = ps->sign_in;
+    a.sign_out = ps->sign_out;
+    a.out_mode = ps->out_mode;
+    a.line_fac = ps->line_fac;
+    a.lf_bstride = ps->line_fac_bstride;
+    a.elem_fac = ps->elem_fac;
+    a.ef_bstride = ps->elem_fac_bstride;
+    a.mask = ps->mask;
+    a.m_bstride = ps->mask_bstride;
+    a.m_rowstride = p->cols;
+    a.chirp = ps->chirp;
+    a.lstep = ps->lstep;
+    a.lcenter = ps->lcenter;
+    a.estep = ps->estep;
+    a.ecenter = ps->ecenter;
+    a.lfft = ps->lfftorder;
+    a.efft = ps->efftorder;
+    a.scale = ps->scale;
+    a.dscale = ps->dscale;
+    a.has_factor = (ps->line_fac || ps->elem_fac || ps->mask || ps->chirp != 0.0 ||
+                    ps->scale != 1.0 || ps->dscale) ? 1 : 0;
+    a.tw = strided ? p->tw_rows : p->tw_cols;
+    return launch_pass(p->prec, n, strided, a, (hipStream_t)stream);
+}
+
+int aw_fft_axis(aw_plan *p, void *data, void *out, int axis, int dir, aw_stream stream) {
+    if (dir != AW_FORWARD && dir != AW_INVERSE)
+        return set_error(AW_ERR_INVALID_VALUE, "direction must be -1 or +1 (got %d)", dir);
+    aw_pass ps{};
+    ps.axis = axis;
+    ps.dir_a = dir;
+    ps.scale = 1.0;
+    ps.out = out;
+    return aw_pass_run(p, data, &ps, stream);
+}
+
+int aw_make_factors(aw_plan *p, void *out, int n, int nvec, int mode, const double *A,
+                    long long a_stride, double ca, const double *B, long long b_stride, double cb,
+                    double c0, const double *mag, long long mag_stride, int mag_pow, double scale,
+                    int alt_sign, aw_stream stream) {
+    if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
+    if (!out || n < 1 || nvec < 1) return set_error(AW_ERR_INVALID_VALUE, "bad factor arguments");
+    if (mode != 0 && mode != 1) return set_error(AW_ERR_INVALID_VALUE, "mode must be 0 or 1");
+    if (mag_pow < 0 || mag_pow > 8) return set_error(AW_ERR_INVALID_VALUE, "mag_pow out of range");
+    FactorArgs f{};
+    f.out = out;
+    f.n = n;
+    f.nvec = nvec;
+    f.mode = mode;
+    f.A = A; f.a_stride = a_stride; f.ca = ca;
+    f.B = B; f.b_stride = b_stride; f.cb = cb;
+    f.c0 = c0;
+    f.mag = mag; f.mag_stride = mag_stride; f.mag_pow = mag_pow;
+    f.scale = scale;
+    f.alt_sign = alt_sign;
+    return launch_make_factors(p->prec, f, (hipStream_t)stream);
+}
+
+int aw_reduce_stats(aw_plan *p, const void *data, double *out, aw_stream stream) {
+    if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
+    if (!data || !out) return set_error(AW_ERR_INVALID_VALUE, "NULL data or out");
+    return launch_stats(p->prec, data, (long long)p->rows * p->cols, p->rows, p->cols, p->batch,
+                        p->partials, p->red_blocks, out, (hipStream_t)stream);
+}
+
+int aw_sum_partials(const double *partials, int nblk, int ncomp, int batch, double *out,
+                    aw_stream stream) {
+    if (!partials || !out || nblk < 1 || ncomp < 1 || batch < 1)
+        return set_error(AW_ERR_INVALID_VALUE, "bad partial-sum arguments");
+    return launch_sum_partials(partials, nblk, ncomp, batch, out, (hipStream_t)stream);
+}
+
+int aw_reduce_vec(int prec, const void *data, long long n, int batch, double *out, aw_stream stream) {
+    if (!data || !out || n < 1 || batch < 1) return set_error(AW_ERR_INVALID_VALUE, "bad vector reduction");
+    if (prec != AW_PREC_F64 && prec != AW_PREC_F32) return set_error(AW_ERR_INVALID_TYPE, "bad precision");
+    long long nb = (n + 256 * 8 - 1) / (256 * 8);
+    if (nb > 1024) nb = 1024;
+    double *partials = nullptr;
+    // small temporary; vector reductions are off the step hot path
+    AW_HIP(hipMallocAsync((void **)&partials, sizeof(double) * 8 * nb * batch, (hipStream_t)stream));
+    int rc = launch_stats(prec, data, n, 1, (int)(n > 0x7fffffff ? 0x7fffffff : n), batch, partials,
+                          (int)nb, out, (hipStream_t)stream);
+    (void)hipFreeAsync(partials, (hipStream_t)stream);
+    return rc;
+}
+
+int aw_scale_from_stats(const double *stats, int stats_stride, int comp, int mode, double factor,
+                        double *dscale, int batch, aw_stream stream) {
+    if (!stats || !dscale || batch < 1) return set_error(AW_ERR_INVALID_VALUE, "bad scale arguments");
+    return launch_scale_from_stats(stats, stats_stride, comp, mode, factor, dscale, batch,
+                                   (hipStream_t)stream);
+}
+
+int aw_scale(int prec, void *data, long long n, int batch, const double *dscale, double scale,
+             aw_stream stream) {
+    if (!data) return set_error(AW_ERR_INVALID_VALUE, "NULL data");
+    return launch_scale(prec, data, n, batch, dscale, scale, (hipStream_t)stream);
+}
+
+int aw_convert(const void *src, int src_prec, void *dst, int dst_prec, long long n, aw_stream stream) {
+    if (!src || !dst) return set_error(AW_ERR_INVALID_VALUE, "NULL pointer");
+    return launch_convert(src, src_prec, dst, dst_prec, n, (hipStream_t)stream);
+}
+
+int aw_mem_info(size_t *free_bytes, size_t *total_bytes) {
+    AW_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return AW_OK;
+}
+
+int aw_sync(aw_stream stream) {
+    if (stream) AW_HIP(hipStreamSynchronize((hipStream_t)stream));
+    else AW_HIP(hipDeviceSynchronize());
+    return AW_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// hipRTC
+// ---------------------------------------------------------------------------------------
+static int rtc_compile(const char *source, const char *name, char *log, size_t log_len,
+                       std::vector<char> &code) {
+    hiprtcProgram prog;
+    hiprtcResult r = hiprtcCreateProgram(&prog, source, name ? name : "awig_rtc.hip", 0, nullptr, nullptr);
+    if (r != HIPRTC_SUCCESS) return set_error(AW_ERR_RTC, "hiprtcCreateProgram: %s", hiprtcGetErrorString(r));
+    const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast"};
+    r = hiprtcCompileProgram(prog, 4, opts);
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    std::string lg(ls + 1, '\0');
+    if (ls) hiprtcGetProgramLog(prog, &lg[0]);
+    if (log && log_len) {
+        std::strncpy(log, lg.c_str(), log_len - 1);
+        log[log_len - 1] = '\0';
+    }
+    if (r != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return set_error(AW_ERR_RTC, "hipRTC compilation failed: %s\n%s", hiprtcGetErrorString(r), lg.c_str());
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    code.resize(cs);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return AW_OK;
+}
+
+int aw_rtc_check(const char *source, char *log, size_t log_len, size_t *code_size) {
+    std::vector<char> code;
+    int rc = rtc_compile(source, "awig_check.hip", log, log_len, code);
+    if (code_size) *code_size = code.size();
+    return rc;
+}
+
+int aw_module_compile(aw_module **mod, const char *source, const char *name, char *log, size_t log_len) {
+    if (!mod || !source) return set_error(AW_ERR_INVALID_VALUE, "NULL module or source");
+    *mod = nullptr;
+    std::vector<char> code;
+    int rc = rtc_compile(source, name, log, log_len, code);
+    if (rc != AW_OK) return rc;
+    aw_module *m = new aw_module();
+    hipError_t e = hipModuleLoadData(&m->mod, code.data());
+    if (e != hipSuccess) {
+        delete m;
+        return set_error(AW_ERR_HIP, "hipModuleLoadData: %s", hipGetErrorString(e));
+    }
+    m->name = name ? name : "";
+    *mod = m;
+    return AW_OK;
+}
+
+int aw_module_destroy(aw_module *m) {
+    if (!m) return set_error(AW_ERR_INVALID_VALUE, "NULL module");
+    if (m->mod) (void)hipModuleUnload(m->mod);
+    delete m;
+    return AW_OK;
+}
+
+int aw_module_launch(aw_module *m, const char *kernel, unsigned gx, unsigned gy, unsigned gz,
+                     unsigned bx, unsigned by, unsigned bz, unsigned shmem, aw_stream stream,
+                     void **args) {
+    if (!m || !kernel) return set_error(AW_ERR_INVALID_VALUE, "NULL module or kernel name");
+    hipFunction_t f;
+    AW_HIP(hipModuleGetFunction(&f, m->mod, kernel));
+    AW_HIP(hipModuleLaunchKernel(f, gx, gy, gz, bx, by, bz, shmem, (hipStream_t)stream, args, nullptr));
+    return AW_OK;
+}
+
+}  // extern "C"

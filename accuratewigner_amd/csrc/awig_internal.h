@@ -1,0 +1,60 @@
+// Internal declarations shared by the libawig translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/awig.h"
+
+namespace awig {
+
+extern thread_local std::string g_err;
+int set_error(int code, const char *fmt, ...);
+
+// kernel-side arguments of one fused line pass (see k_line_pass)
+struct PassArgs {
+    const void *in;
+    void *out;
+    long long bstride;      // elements between batch items
+    long long line_stride;  // elements between consecutive lines
+    long long elem_stride;  // elements between consecutive elements of a line
+    long long total_lines;  // nlines * batch
+    int nlines;             // lines per batch item
+    int dir_a, dir_b;
+    int sign_in, sign_out, out_mode;
+    int has_factor;
+    const void *line_fac; long long lf_bstride;
+    const void *elem_fac; long long ef_bstride;
+    const void *mask; long long m_bstride; long long m_rowstride;
+    double chirp, lstep, lcenter, estep, ecenter;
+    int lfft, efft;
+    double scale;
+    const double *dscale;
+    const void *tw;         // N roots exp(-2 pi i t / N), plan precision
+};
+
+struct FactorArgs {
+    void *out;
+    int n, nvec, mode;
+    const double *A; long long a_stride; double ca;
+    const double *B; long long b_stride; double cb;
+    double c0;
+    const double *mag; long long mag_stride; int mag_pow;
+    double scale;
+    int alt_sign;
+};
+
+int launch_pass(int prec, int n, bool strided, const PassArgs &a, hipStream_t st);
+int pass_lines_per_block(int prec, int n, bool strided);
+int launch_make_factors(int prec, const FactorArgs &f, hipStream_t st);
+int launch_stats(int prec, const void *data, long long n_item, int rows, int cols, int batch,
+                 double *partials, int nblk, double *out, hipStream_t st);
+int launch_sum_partials(const double *partials, int nblk, int ncomp, int batch, double *out,
+                        hipStream_t st);
+int launch_scale_from_stats(const double *stats, int stride, int comp, int mode, double factor,
+                            double *dscale, int batch, hipStream_t st);
+int launch_scale(int prec, void *data, long long n, int batch, const double *dscale, double scale,
+                 hipStream_t st);
+int launch_convert(const void *src, int sp, void *dst, int dp, long long n, hipStream_t st);
+
+}  // namespace awig

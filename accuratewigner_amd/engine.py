@@ -1,0 +1,146 @@
+"""Host-side wrapper of libawig plans: fused passes, factor vectors and reductions.
+
+A ``Plan`` owns one ``aw_plan`` (twiddles + reduction scratch) for a batch of
+[rows, cols] grids in one precision.  Everything runs on torch's current HIP stream and
+never synchronises with the host unless a caller reads a result.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceArray, current_stream, empty
+
+PRECISIONS = {"fp64": _lib.AW_PREC_F64, "mixed": _lib.AW_PREC_F32, "fp32": _lib.AW_PREC_F32}
+STATE_DTYPE = {_lib.AW_PREC_F64: np.complex128, _lib.AW_PREC_F32: np.complex64}
+REAL_DTYPE = {_lib.AW_PREC_F64: np.float64, _lib.AW_PREC_F32: np.float32}
+
+FWD, INV = _lib.AW_FORWARD, _lib.AW_INVERSE
+
+
+def _p(x):
+    """Device pointer of a DeviceArray / tensor / int / None."""
+    if x is None:
+        return None
+    if isinstance(x, DeviceArray):
+        return x.ptr
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+class Plan:
+    """Batched [rows, cols] engine (replaces the cuFFT plans Plan_Z2Z_2D_Axis0/1,
+    cufft.py:274-345, and the PyCUDA elementwise kernels around them)."""
+
+    def __init__(self, rows, cols, batch=1, precision="fp64"):
+        self.rows, self.cols, self.batch = int(rows), int(cols), int(batch)
+        self.prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self.state_dtype = STATE_DTYPE[self.prec]
+        self.real_dtype = REAL_DTYPE[self.prec]
+        h = ctypes.c_void_p()
+        _lib.call("aw_plan_create", ctypes.byref(h), self.rows, self.cols, self.batch, self.prec)
+        self.handle = h
+        self._stats = empty((self.batch, 8), np.float64)
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                _lib.load().aw_plan_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    # -- passes ------------------------------------------------------------------------
+    def run(self, data, axis, dir_a=0, dir_b=0, line_fac=None, elem_fac=None, mask=None,
+            line_bstride=0, elem_bstride=0, mask_bstride=0, sign_in=0, sign_out=0,
+            chirp=0.0, lstep=0.0, lcenter=0.0, estep=0.0, ecenter=0.0, lfft=0, efft=0,
+            scale=1.0, dscale=None, out=None, out_mode=0, stream=None):
+        ps = _lib.aw_pass()
+        ps.axis, ps.dir_a, ps.dir_b = axis, dir_a, dir_b
+        ps.sign_in, ps.sign_out, ps.out_mode = sign_in, sign_out, out_mode
+        ps.line_fac, ps.line_fac_bstride = _p(line_fac), line_bstride
+        ps.elem_fac, ps.elem_fac_bstride = _p(elem_fac), elem_bstride
+        ps.mask, ps.mask_bstride = _p(mask), mask_bstride
+        ps.chirp, ps.lstep, ps.lcenter, ps.estep, ps.ecenter = chirp, lstep, lcenter, estep, ecenter
+        ps.lfftorder, ps.efftorder = lfft, efft
+        ps.scale = scale
+        ps.dscale = _p(dscale)
+        ps.out = _p(out)
+        _lib.call("aw_pass_run", self.handle, ctypes.c_void_p(_p(data)), ctypes.byref(ps),
+                  ctypes.c_void_p(stream if stream is not None else current_stream()))
+
+    def fft(self, data, axis, direction, out=None):
+        _lib.call("aw_fft_axis", self.handle, ctypes.c_void_p(_p(data)),
+                  ctypes.c_void_p(_p(out)), axis, direction, ctypes.c_void_p(current_stream()))
+
+    # -- factors -----------------------------------------------------------------------
+    def factors(self, out, n, nvec, A=None, ca=0.0, B=None, cb=0.0, c0=0.0, mag=None,
+                mag_pow=1, scale=1.0, alt_sign=False, mode=0, a_stride=None, b_stride=None,
+                mag_stride=None, out_offset=0):
+        """out[v][k] = scale * (-1)^k? * mag^pow * E(c0 + ca A + cb B), E = exp(i.) (mode 0)
+        or exp(.) (mode 1); A, B, mag are FP64 device pointers (ints) or None."""
+        esz = np.dtype(self.state_dtype).itemsize
+        _lib.call("aw_make_factors", self.handle, ctypes.c_void_p(_p(out) + out_offset * n * esz),
+                  n, nvec, mode,
+                  ctypes.c_void_p(A), n if a_stride is None else a_stride, ca,
+                  ctypes.c_void_p(B), n if b_stride is None else b_stride, cb,
+                  c0, ctypes.c_void_p(mag), 0 if mag_stride is None else mag_stride, mag_pow,
+                  scale, int(bool(alt_sign)), ctypes.c_void_p(current_stream()))
+
+    def new_factors(self, n, nvec):
+        return empty((nvec, n), self.state_dtype)
+
+    # -- reductions --------------------------------------------------------------------
+    def stats(self, data, out=None):
+        """[batch, 8] device FP64: Re/Im sum, sum |z|^2, Re/Im sum z^2, Re/Im trace, 0."""
+        out = out if out is not None else self._stats
+        _lib.call("aw_reduce_stats", self.handle, ctypes.c_void_p(_p(data)), ctypes.c_void_p(_p(out)),
+                  ctypes.c_void_p(current_stream()))
+        return out
+
+    def scale_from_stats(self, stats_ptr, stride, comp, mode, factor, dscale, batch=None):
+        _lib.call("aw_scale_from_stats", ctypes.c_void_p(stats_ptr), stride, comp, mode, factor,
+                  ctypes.c_void_p(_p(dscale)), self.batch if batch is None else batch,
+                  ctypes.c_void_p(current_stream()))
+
+    def scale(self, data, dscale=None, scale=1.0, n=None):
+        n = self.rows * self.cols if n is None else n
+        _lib.call("aw_scale", self.prec, ctypes.c_void_p(_p(data)), n, self.batch,
+                  ctypes.c_void_p(_p(dscale)), scale, ctypes.c_void_p(current_stream()))
+
+
+def sum_partials(partials, nblk, ncomp, batch, out):
+    _lib.call("aw_sum_partials", ctypes.c_void_p(_p(partials)), nblk, ncomp, batch,
+              ctypes.c_void_p(_p(out)), ctypes.c_void_p(current_stream()))
+
+
+def reduce_vec(prec, data, n, batch, out):
+    _lib.call("aw_reduce_vec", prec, ctypes.c_void_p(_p(data)), n, batch, ctypes.c_void_p(_p(out)),
+              ctypes.c_void_p(current_stream()))
+
+
+def convert(src, src_prec, dst, dst_prec, n):
+    _lib.call("aw_convert", ctypes.c_void_p(_p(src)), src_prec, ctypes.c_void_p(_p(dst)), dst_prec,
+              n, ctypes.c_void_p(current_stream()))
+
+
+def mem_info():
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    _lib.call("aw_mem_info", ctypes.byref(f), ctypes.byref(t))
+    return f.value, t.value
+
+
+def rank1_split(M, rtol=1e-13):
+    """If the real table M[i, j] equals v[i] * u[j] (to rtol of max|M|), return (v, u), else
+    None.  Used to turn a separable absorbing-mask expression into two vectors."""
+    M = np.asarray(M, dtype=np.float64)
+    amax = np.max(np.abs(M))
+    if amax == 0.0:
+        return np.zeros(M.shape[0]), np.zeros(M.shape[1])
+    i0, j0 = np.unravel_index(np.argmax(np.abs(M)), M.shape)
+    u = M[i0, :].copy()
+    v = M[:, j0] / M[i0, j0]
+    if np.max(np.abs(np.outer(v, u) - M)) <= rtol * amax:
+        return v, u
+    return None

@@ -1,0 +1,542 @@
+"""WignerMoyalCUDA1D: drop-in MI355X implementation of the reference class of the same name
+(wigner_moyal_cuda_1d.py:10-828).
+
+Same constructor keywords, attributes and methods.  Time stepping follows the reference's
+operator sequence (wigner_moyal_cuda_1d.py:543-561):
+
+    rho <- EV rho; I0; F1; EK; F0; I1; /(Nx Np); EV
+
+but runs as fused line passes: a column pass [F0 -> EV(t_n) EV(t_{n+1}) -> I0] (the two
+half-kicks of consecutive steps merged) and a row pass [F1 -> EK -> I1] per step, so each
+step reads and writes the state twice.  Phase factors are outer products of 1D vectors
+evaluated in FP64 from the user's C expressions (hipRTC), exactly the expV / expK phases
+-0.5 dt [V(XX_j) - V(XX'_i)] and -dt [K(PP_j) - K(PP'_i)] at t + dt/2 (:751, :772).
+
+Extra keyword (not in the reference): ``precision`` = "fp64" (default, complex128 state,
+the reference's arithmetic) or "mixed" (complex64 state and FFT arithmetic, FP64 phase
+arguments and FP64 reductions).
+"""
+import ctypes
+from types import FunctionType, MethodType
+
+import numpy as np
+
+from . import _lib, rtc
+from .device import DeviceArray, current_stream, empty, require_gpu, to_device
+from .engine import FWD, INV, Plan, mem_info, rank1_split, sum_partials
+
+SQRT2 = np.sqrt(2.0)
+CHUNK_STEPS = 256
+
+
+class WignerMoyalCUDA1D:
+    """The second-order split-operator propagator for the Moyal equation for the Wigner
+    function W(x, p, t) with the time-dependent Hamiltonian H = K(p, t) + V(x, t)
+    (reference: wigner_moyal_cuda_1d.py:10-16), on MI355X."""
+
+    # which C-expression observables the Ehrenfest path records (get_Ehrenfest, :584-606)
+    def __init__(self, **kwargs):
+        """Keywords as in the reference constructor (wigner_moyal_cuda_1d.py:17-41):
+        X_gridDIM, X_amplitude, P_gridDIM, P_amplitude, V, K, dt, and optionally t, consts,
+        functions, diff_V, diff_K, abs_boundary_x, abs_boundary_p, max_thread_block, plus
+        ``precision`` ("fp64" | "mixed")."""
+        kwargs = dict(kwargs)
+        self.precision = kwargs.pop("precision", "fp64")
+        # save all attributes; Python functions become bound methods (:44-50)
+        for name, value in kwargs.items():
+            if isinstance(value, FunctionType):
+                setattr(self, name, MethodType(value, self))
+            else:
+                setattr(self, name, value)
+
+        # required-parameter checks (:53-92)
+        if not hasattr(self, "X_gridDIM"):
+            raise AttributeError("Coordinate grid size (X_gridDIM) was not specified")
+        assert 2 ** int(np.log2(self.X_gridDIM)) == self.X_gridDIM, \
+            "Coordinate grid size (X_gridDIM) must be a power of two"
+        if not hasattr(self, "P_gridDIM"):
+            raise AttributeError("Momentum grid size (P_gridDIM) was not specified")
+        assert 2 ** int(np.log2(self.P_gridDIM)) == self.P_gridDIM, \
+            "Momentum grid size (P_gridDIM) must be a power of two"
+        if not hasattr(self, "X_amplitude"):
+            raise AttributeError("Coordinate grid range (X_amplitude) was not specified")
+        if not hasattr(self, "P_amplitude"):
+            raise AttributeError("Momentum grid range (P_amplitude) was not specified")
+        if not hasattr(self, "V"):
+            raise AttributeError("Potential energy (V) was not specified")
+        if not hasattr(self, "K"):
+            raise AttributeError("Momentum dependence (K) was not specified")
+        if not hasattr(self, "dt"):
+            raise AttributeError("Time-step (dt) was not specified")
+        if hasattr(self, "t"):
+            kwargs.pop("t", None)
+        else:
+            print("Warning: Initial time (t) was not specified, thus it is set to zero.")
+            self.t = 0.
+        self.t = np.float64(self.t)
+
+        self._make_grids()
+
+        # #define block (:151-182)
+        kwargs.update(
+            dX=self.dX, dP=self.dP, dXX=self.dXX, dXX_prime=self.dXX_prime, dPP=self.dPP,
+            dPP_prime=self.dPP_prime, XX_max=self.XX.max(), XX_prime_max=self.XX_prime.max(),
+            PP_max=self.PP.max(), PP_prime_max=self.PP_prime.max())
+        self._defines = kwargs
+        self.cuda_consts = rtc.cuda_consts(kwargs, getattr(self, "consts", None), None)
+        if not hasattr(self, "abs_boundary_x"):
+            self.abs_boundary_x = "1."
+        if not hasattr(self, "abs_boundary_p"):
+            self.abs_boundary_p = "1."
+        if not hasattr(self, "max_thread_block"):
+            self.max_thread_block = 512
+        self.cuda_consts += getattr(self, "functions", "") or ""
+
+        require_gpu()
+        self.plan = Plan(self.P_gridDIM, self.X_gridDIM, 1, self.precision)
+        self.state_dtype = self.plan.state_dtype
+        self._compile_physics()
+
+        # state arrays (:268-271); W and rho in the chosen precision
+        shape = (self.P_gridDIM, self.X_gridDIM)
+        self.wignerfunction = DeviceArray(shape, self.state_dtype)
+        self.rho = DeviceArray(shape, self.state_dtype)
+        self._dscale = empty((1, 2), np.float64)
+        self._scale_pending = False
+
+        self.hamiltonian = self.K + ' + ' + self.V
+        self.isEhrenfest = hasattr(self, "diff_K") and hasattr(self, "diff_V")
+        if self.isEhrenfest:
+            self.X_average, self.P_average = [], []
+            self.X_average_RHS, self.P_average_RHS = [], []
+            self.hamiltonian_average = []
+        self._observable_kernels = {}
+        self._prepare_masks()
+        self.print_memory_info()
+
+    # ------------------------------------------------------------------------------------
+    # grids and compiled physics
+    # ------------------------------------------------------------------------------------
+    def _make_grids(self):
+        """Grids exactly as wigner_moyal_cuda_1d.py:110-143."""
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        Xa, Pa = self.X_amplitude, self.P_amplitude
+        self.dX = 2. * Xa / Nx
+        self.dP = 2. * Pa / Np
+        self.dXdP = self.dX * self.dP
+        self.X = np.linspace(-Xa, Xa - self.dX, Nx)
+        self.P = np.linspace(-Pa, Pa - self.dP, Np)
+        self.PP_prime = SQRT2 * self.P
+        self.dPP_prime = self.PP_prime[1] - self.PP_prime[0]
+        self.XX_prime = np.arange(Np, dtype=np.float64) - Np // 2
+        self.XX_prime *= np.pi / (SQRT2 * Pa)
+        self.dXX_prime = self.XX_prime[1] - self.XX_prime[0]
+        self.XX = SQRT2 * self.X
+        self.dXX = self.XX[1] - self.XX[0]
+        self.PP = np.arange(Nx, dtype=np.float64) - Nx // 2
+        self.PP *= np.pi / (SQRT2 * Xa)
+        self.dPP = self.PP[1] - self.PP[0]
+        self.P = self.P[:, np.newaxis]
+        self.X = self.X[np.newaxis, :]
+        self.XX = self.XX[np.newaxis, :]
+        self.PP = self.PP[np.newaxis, :]
+        self.XX_prime = self.XX_prime[:, np.newaxis]
+        self.PP_prime = self.PP_prime[:, np.newaxis]
+        # coordinates as the reference kernels recompute them from the step constants
+        j = np.arange(Nx, dtype=np.float64)
+        i = np.arange(Np, dtype=np.float64)
+        self._kXX = self.dXX * (j - 0.5 * Nx)
+        self._kXXp = self.dXX_prime * (i - 0.5 * Np)
+        self._kPP = self.dPP * (j - 0.5 * Nx)
+        self._kPPp = self.dPP_prime * (i - 0.5 * Np)
+
+    def _state_t(self):
+        return "double2" if self.state_dtype == np.complex128 else "float2"
+
+    def _physics_source(self):
+        return (rtc.PRELUDE + self.cuda_consts
+                + "\n__device__ double K(double P, double t)\n{\n    return (%s);\n}\n" % self.K
+                + "\n__device__ double V(double X, double t)\n{\n    return (%s);\n}\n" % self.V
+                + rtc.EVAL_1D
+                + r"""
+__device__ double aw_absx(double XX, double XX_prime, double t, size_t i, size_t j)
+{
+    return (%s);
+}
+__device__ double aw_absp(double PP, double PP_prime, double t, size_t i, size_t j)
+{
+    return (%s);
+}
+extern "C" __global__ void aw_eval_mask(int which, const double *xj, int nj, const double *xi,
+                                        int ni, double t, double *out)
+{
+    const long long total = (long long)ni * nj;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const size_t i = idx / nj, j = idx %% nj;
+        out[idx] = which == 0 ? aw_absx(xj[j], xi[i], t, i, j) : aw_absp(xj[j], xi[i], t, i, j);
+    }
+}
+""" % (self.abs_boundary_x, self.abs_boundary_p))
+
+    def _compile_physics(self):
+        self._mod = rtc.Module(self._physics_source(), "wigner_moyal_physics.hip")
+        self._dev_kXX = to_device(self._kXX)
+        self._dev_kXXp = to_device(self._kXXp)
+        self._dev_kPP = to_device(self._kPP)
+        self._dev_kPPp = to_device(self._kPPp)
+
+    def _eval_1d(self, which, xdev, n, ts_dev, nt, tshift, out):
+        blocks, threads = rtc.launch_cfg(n * nt)
+        self._mod.launch("aw_eval_1d", (blocks,), (threads,),
+                         [ctypes.c_int(which), xdev.ptr, ctypes.c_int(n), ts_dev.ptr,
+                          ctypes.c_int(nt), ctypes.c_double(tshift), out.ptr], current_stream())
+
+    def _eval_mask(self, which, t):
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        out = empty((Np, Nx), np.float64)
+        xj, xi = (self._dev_kXX, self._dev_kXXp) if which == 0 else (self._dev_kPP, self._dev_kPPp)
+        blocks, threads = rtc.launch_cfg(Nx * Np)
+        self._mod.launch("aw_eval_mask", (blocks,), (threads,),
+                         [ctypes.c_int(which), xj.ptr, ctypes.c_int(Nx), xi.ptr, ctypes.c_int(Np),
+                          ctypes.c_double(t), out.ptr], current_stream())
+        return out.get()
+
+    def _prepare_masks(self):
+        """Evaluate abs_boundary_x(XX, XX') and abs_boundary_p(PP, PP') once (they are
+        applied in every expV / expK, :753, :777).  A separable mask becomes two vectors;
+        a non-separable one a 2D table.  Masks that depend on t are rejected."""
+        self._masks = {}
+        for which, expr in ((0, self.abs_boundary_x), (1, self.abs_boundary_p)):
+            if expr.strip() in ("1.", "1", "1.0"):
+                self._masks[which] = None
+                continue
+            M0 = self._eval_mask(which, float(self.t))
+            M1 = self._eval_mask(which, float(self.t) + 1.2345)
+            if not np.array_equal(M0, M1):
+                raise NotImplementedError(
+                    "time-dependent absorbing boundaries are not supported (%r)" % expr)
+            split = rank1_split(M0)
+            if split is not None:
+                v, u = split
+                self._masks[which] = ("vec", to_device(v), to_device(u))
+            else:
+                rd = self.plan.real_dtype
+                self._masks[which] = ("table", to_device(M0.astype(rd)),
+                                      to_device((M0 * M0).astype(rd)))
+
+    # ------------------------------------------------------------------------------------
+    # observables
+    # ------------------------------------------------------------------------------------
+    def _observable_kernel(self, exprs):
+        key = tuple(exprs)
+        mod = self._observable_kernels.get(key)
+        if mod is None:
+            nobs = len(exprs)
+            body = "".join("        acc[%d] += w * (%s);\n" % (k, e) for k, e in enumerate(exprs))
+            src = (rtc.PRELUDE + self.cuda_consts
+                   + "\n__device__ double K(double P, double t)\n{\n    return (%s);\n}\n" % self.K
+                   + "\n__device__ double V(double X, double t)\n{\n    return (%s);\n}\n" % self.V
+                   + r"""
+extern "C" __global__ void aw_obs(const %(st)s *W, long long n_item, double t, double *partials)
+{
+    double acc[%(nobs)d];
+    for (int c = 0; c < %(nobs)d; ++c) acc[c] = 0.;
+    const %(st)s *Wb = W + (long long)blockIdx.y * n_item;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n_item;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const size_t i = idx / X_gridDIM, j = idx %% X_gridDIM;
+        const double X = dX * (j - 0.5 * X_gridDIM);
+        const double P = dP * (i - 0.5 * P_gridDIM);
+        const double w = (double)Wb[idx].x;
+%(body)s    }
+""" % dict(st=self._state_t(), nobs=nobs, body=body) + rtc.REDUCE_TAIL % dict(nobs=nobs) + "}\n")
+            mod = rtc.Module(src, "wigner_moyal_observables.hip")
+            self._observable_kernels[key] = mod
+        return mod
+
+    def _obs_launch(self, exprs, W, t, out):
+        """Partial sums of Re W * f_k(X, P, t) for each expression, summed into out[k]."""
+        mod = self._observable_kernel(exprs)
+        n = self.X_gridDIM * self.P_gridDIM
+        nblk = max(1, min(512, n // 2048))
+        partials = empty((nblk, len(exprs)), np.float64)
+        mod.launch("aw_obs", (nblk, 1), (256,), [W.ptr, ctypes.c_longlong(n), ctypes.c_double(t),
+                                                 partials.ptr], current_stream())
+        sum_partials(partials, nblk, len(exprs), 1, out)
+
+    def get_average(self, observable):
+        """Sum Re W * f(X, P, t) dXdP (wigner_moyal_cuda_1d.py:329-347)."""
+        out = empty((1,), np.float64)
+        self._obs_launch([observable], self.wignerfunction, float(self.t), out)
+        return float(out.get()[0]) * self.dXdP
+
+    def get_purity(self):
+        """2 pi Re dot(W, W) dXdP without conjugation (wigner_moyal_cuda_1d.py:349-354)."""
+        s = self.plan.stats(self.wignerfunction).get()[0]
+        return 2. * np.pi * s[3] * self.dXdP
+
+    def get_sigma_x_sigma_p(self):
+        """wigner_moyal_cuda_1d.py:356-366."""
+        return np.sqrt((self.get_average("X * X") - self.get_average("X") ** 2)
+                       * (self.get_average("P * P") - self.get_average("P") ** 2))
+
+    @classmethod
+    def print_memory_info(cls):
+        """wigner_moyal_cuda_1d.py:368-377."""
+        free, total = mem_info()
+        print("\n\n\t\tGPU memory Total %.2f GB\n\t\tGPU memory Free %.2f GB\n"
+              % (total / 2. ** 30, free / 2. ** 30))
+
+    # ------------------------------------------------------------------------------------
+    # initial state and Wigner <-> rho transforms
+    # ------------------------------------------------------------------------------------
+    def _init_kernel_source(self, expr):
+        return (rtc.PRELUDE + self.cuda_consts + r"""
+extern "C" __global__ void aw_init(%(st)s *W, long long n_item)
+{
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n_item;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const size_t i = idx / X_gridDIM, j = idx %% X_gridDIM;
+        const double X = dX * (j - 0.5 * X_gridDIM);
+        const double P = dP * (i - 0.5 * P_gridDIM);
+        aw_state_store<%(st)s>::put(W + idx, cuda_complex(%(expr)s));
+    }
+}
+""" % dict(st=self._state_t(), expr=expr))
+
+    def set_wignerfunction(self, new_wigner_func):
+        """wigner_moyal_cuda_1d.py:379-420: array, GPU array, Python function F(self, X, P),
+        C string (X, P declared) or number; then W /= (complex) sum W dXdP and wigner2rho."""
+        if isinstance(new_wigner_func, (np.ndarray, DeviceArray)):
+            assert new_wigner_func.shape == self.wignerfunction.shape, \
+                "The grid sizes does not match with the Wigner function"
+            self.wignerfunction[:] = new_wigner_func
+        elif isinstance(new_wigner_func, FunctionType):
+            self.wignerfunction[:] = np.asarray(new_wigner_func(self, self.X, self.P),
+                                                dtype=np.complex128) * np.ones(self.wignerfunction.shape)
+        elif isinstance(new_wigner_func, str):
+            mod = rtc.Module(self._init_kernel_source(new_wigner_func), "wigner_moyal_init.hip")
+            n = self.X_gridDIM * self.P_gridDIM
+            blocks, threads = rtc.launch_cfg(n)
+            mod.launch("aw_init", (blocks,), (threads,), [self.wignerfunction.ptr, ctypes.c_longlong(n)],
+                       current_stream())
+        elif isinstance(new_wigner_func, (float, complex, int)):
+            self.wignerfunction.fill(np.complex128(new_wigner_func))
+        else:
+            raise NotImplementedError("new_wigner_func must be either function or numpy.array")
+        # W /= sum(W) dXdP with the complex sum (:415)
+        st = self.plan.stats(self.wignerfunction)
+        self.plan.scale_from_stats(st.ptr, 8, 0, 1, self.dXdP, self._dscale)
+        self.plan.scale(self.wignerfunction, self._dscale)
+        self._scale_pending = False
+        self.wigner2rho()
+        return self
+
+    def _shearX(self, data, dir_a, dir_b, **kw):
+        """phase_shearX (:639-654): exp(-i tan(pi/8) PP_j XX'_i) / Nx on row lines."""
+        self.plan.run(data, 1, dir_a, dir_b, chirp=-np.tan(np.pi / 8.),
+                      lstep=self.dXX_prime, lcenter=0.5 * self.P_gridDIM,
+                      estep=self.dPP, ecenter=0.5 * self.X_gridDIM,
+                      scale=1.0 / self.X_gridDIM, **kw)
+
+    def _shearY(self, data, dir_a, dir_b, **kw):
+        """phase_shearY (:656-670): exp(+i sin(pi/4) PP'_i XX_j) / Np on column lines."""
+        self.plan.run(data, 0, dir_a, dir_b, chirp=np.sin(np.pi / 4.),
+                      lstep=self.dXX, lcenter=0.5 * self.X_gridDIM,
+                      estep=self.dPP_prime, ecenter=0.5 * self.P_gridDIM,
+                      scale=1.0 / self.P_gridDIM, **kw)
+
+    def wigner2rho(self):
+        """wigner_moyal_cuda_1d.py:422-469 as four passes:
+        [S, F0] [I1, SX, F1] [I0, SY, F0] [I1, SX, F1, S]."""
+        self.plan.run(self.wignerfunction, 0, FWD, 0, sign_in=1, out=self.rho)
+        self._shearX(self.rho, INV, FWD)
+        self._shearY(self.rho, INV, FWD)
+        self._shearX(self.rho, INV, FWD, sign_out=1)
+        return self.rho
+
+    def rho2wigner_(self):
+        """wigner_moyal_cuda_1d.py:471-501 as four passes:
+        [S, F1, SX, I1] [F0, SY, I0] [F1, SX, I1] [I0, S]."""
+        self._flush_scale()
+        self._shearX(self.rho, FWD, INV, sign_in=1, out=self.wignerfunction)
+        self._shearY(self.wignerfunction, FWD, INV)
+        self._shearX(self.wignerfunction, FWD, INV)
+        self.plan.run(self.wignerfunction, 0, INV, 0, sign_out=1)
+        return self.wignerfunction
+
+    rho2wigner = rho2wigner_
+
+    def rho2wigner_blackman(self):
+        """wigner_moyal_cuda_1d.py:503-538 (Blackman windows before each forward FFT)."""
+        self._flush_scale()
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        bx = self._blackman(Nx)
+        by = self._blackman(Np)
+        # [S, Bx] then [F1, SX, I1]: the window multiplies before the FFT, so apply it in a
+        # plain pass first
+        self.plan.run(self.rho, 1, 0, 0, sign_in=1, elem_fac=bx, out=self.wignerfunction)
+        self._shearX(self.wignerfunction, FWD, INV)
+        self.plan.run(self.wignerfunction, 0, 0, 0, elem_fac=by)
+        self._shearY(self.wignerfunction, FWD, INV)
+        self.plan.run(self.wignerfunction, 1, 0, 0, elem_fac=bx)
+        self._shearX(self.wignerfunction, FWD, INV)
+        self.plan.run(self.wignerfunction, 0, 0, INV, elem_fac=by, sign_out=1)
+        return self.wignerfunction
+
+    def _blackman(self, n):
+        """blackmanX / blackmanY (:678-698) as a complex vector in the state precision."""
+        c = np.arange(n) * 2.0 * np.pi / (n - 1.)
+        return to_device((0.42 - 0.5 * np.cos(c) + 0.08 * np.cos(2.0 * c)).astype(self.state_dtype))
+
+    def _flush_scale(self):
+        """Apply a pending device-side normalisation to rho."""
+        if self._scale_pending:
+            self.plan.scale(self.rho, self._dscale)
+            self._scale_pending = False
+
+    def normalize_rho_wigner(self):
+        """wigner_moyal_cuda_1d.py:608-622: W = rho2wigner; n = Re sum W dXdP; W /= n;
+        rho /= n (device-side, no host sync)."""
+        self.rho2wigner()
+        st = self.plan.stats(self.wignerfunction)
+        self.plan.scale_from_stats(st.ptr, 8, 0, 0, self.dXdP, self._dscale)
+        self.plan.scale(self.wignerfunction, self._dscale)
+        self.plan.scale(self.rho, self._dscale)
+        return self
+
+    # ------------------------------------------------------------------------------------
+    # propagation
+    # ------------------------------------------------------------------------------------
+    def _times(self, steps):
+        """t after each ``t += dt`` (:571), accumulated in float64 like the reference."""
+        ts = np.empty(steps, np.float64)
+        t = self.t
+        for k in range(steps):
+            t = t + self.dt
+            ts[k] = t
+        return ts
+
+    def _mask_args(self, which, power):
+        m = self._masks.get(which)
+        if m is None:
+            return None, None, None
+        if m[0] == "vec":
+            return m[1], m[2], None          # (v over lines-i, u over cols-j)
+        return None, None, (m[1] if power == 1 else m[2])
+
+    def _build_chunk_factors(self, ts, first_dscale):
+        """Phase tables (FP64, hipRTC) and the per-pass complex factor vectors of one chunk."""
+        n = len(ts)
+        Nx, Np, dt = self.X_gridDIM, self.P_gridDIM, self.dt
+        p = self.plan
+        ts_dev = to_device(ts)
+        Vj = empty((n, Nx), np.float64)
+        Vi = empty((n, Np), np.float64)
+        Kj = empty((n, Nx), np.float64)
+        Ki = empty((n, Np), np.float64)
+        self._eval_1d(0, self._dev_kXX, Nx, ts_dev, n, 0.5 * dt, Vj)
+        self._eval_1d(0, self._dev_kXXp, Np, ts_dev, n, 0.5 * dt, Vi)
+        self._eval_1d(1, self._dev_kPP, Nx, ts_dev, n, 0.5 * dt, Kj)
+        self._eval_1d(1, self._dev_kPPp, Np, ts_dev, n, 0.5 * dt, Ki)
+        mxp, mx, _ = self._mask_args(0, 1)
+        mpp, mp, _ = self._mask_args(1, 1)
+        mxp_p = mxp.ptr if mxp is not None else None
+        mx_p = mx.ptr if mx is not None else None
+        colL = p.new_factors(Nx, n + 1)
+        colE = p.new_factors(Np, n + 1)
+        rowL = p.new_factors(Np, n)
+        rowE = p.new_factors(Nx, n)
+        # column passes: v = 0 start EV(t_0) (with S), 1..n-1 merged EV EV', n end EV (with S)
+        p.factors(colL, Nx, 1, A=Vj.ptr, ca=-0.5 * dt, mag=mx_p, alt_sign=True)
+        p.factors(colE, Np, 1, A=Vi.ptr, ca=0.5 * dt, mag=mxp_p, alt_sign=True)
+        if n > 1:
+            p.factors(colL, Nx, n - 1, A=Vj.ptr, ca=-0.5 * dt, B=Vj.ptr + Nx * 8, cb=-0.5 * dt,
+                      mag=mx_p, mag_pow=2, out_offset=1)
+            p.factors(colE, Np, n - 1, A=Vi.ptr, ca=0.5 * dt, B=Vi.ptr + Np * 8, cb=0.5 * dt,
+                      mag=mxp_p, mag_pow=2, scale=1.0 / Np, out_offset=1)
+        p.factors(colL, Nx, 1, A=Vj.ptr + (n - 1) * Nx * 8, ca=-0.5 * dt, mag=mx_p, alt_sign=True,
+                  out_offset=n)
+        p.factors(colE, Np, 1, A=Vi.ptr + (n - 1) * Np * 8, ca=0.5 * dt, mag=mxp_p, alt_sign=True,
+                  scale=1.0 / Np, out_offset=n)
+        # row passes: exp(-i dt [K(PP_j) - K(PP'_i)]) absp / Nx
+        p.factors(rowL, Np, n, A=Ki.ptr, ca=dt, mag=mpp.ptr if mpp is not None else None,
+                  scale=1.0 / Nx)
+        p.factors(rowE, Nx, n, A=Kj.ptr, ca=-dt, mag=mp.ptr if mp is not None else None)
+        return colL, colE, rowL, rowE
+
+    def _run_chunk(self, ts, dscale=None):
+        """Steps with evaluation times ts (t after the increment) as 2 n + 1 fused passes."""
+        n = len(ts)
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        colL, colE, rowL, rowE = self._build_chunk_factors(ts, dscale)
+        csz = np.dtype(self.state_dtype).itemsize
+        _, _, tab1 = self._mask_args(0, 1)
+        _, _, tab2 = self._mask_args(0, 2)
+        _, _, tabp = self._mask_args(1, 1)
+        p, rho = self.plan, self.rho
+        p.run(rho, 0, 0, INV, line_fac=colL.ptr, elem_fac=colE.ptr, mask=tab1, dscale=dscale)
+        for s in range(n):
+            p.run(rho, 1, FWD, INV, line_fac=rowL.ptr + s * Np * csz, elem_fac=rowE.ptr + s * Nx * csz,
+                  mask=tabp)
+            last = s == n - 1
+            p.run(rho, 0, FWD, 0 if last else INV, line_fac=colL.ptr + (s + 1) * Nx * csz,
+                  elem_fac=colE.ptr + (s + 1) * Np * csz, mask=tab1 if last else tab2)
+
+    def single_step_propagation(self):
+        """wigner_moyal_cuda_1d.py:543-561 (unnormalised result in self.rho)."""
+        self._flush_scale()
+        self._run_chunk(np.array([self.t], np.float64))
+        return self.rho
+
+    def propagate(self, steps=1):
+        """wigner_moyal_cuda_1d.py:563-582: t += dt before each step; Ehrenfest lists every
+        step when diff_K and diff_V are given, otherwise one normalisation at the end."""
+        steps = int(steps)
+        if steps <= 0:
+            if not self.isEhrenfest:
+                self.normalize_rho_wigner()
+            return self.wignerfunction
+        ts = self._times(steps)
+        if not self.isEhrenfest:
+            self._flush_scale()
+            for c0 in range(0, steps, CHUNK_STEPS):
+                self._run_chunk(ts[c0:c0 + CHUNK_STEPS])
+            self.t = ts[-1]
+            self.normalize_rho_wigner()
+            return self.wignerfunction
+        # Ehrenfest: per step normalise through the Wigner transform and record
+        # <X>, <diff_K>, <P>, -<diff_V>, <K + V> (:584-606) into a device ring; one host copy
+        exprs = ["1.", "X", self.diff_K, "P", self.diff_V, self.hamiltonian]
+        ring = empty((steps, len(exprs)), np.float64)
+        for s in range(steps):
+            self._run_chunk(ts[s:s + 1], dscale=self._dscale if self._scale_pending else None)
+            self._scale_pending = False
+            self.rho2wigner_()
+            self._obs_launch(exprs, self.wignerfunction, float(ts[s]), ring.ptr + s * len(exprs) * 8)
+            self.plan.scale_from_stats(ring.ptr + s * len(exprs) * 8, len(exprs), 0, 0, self.dXdP,
+                                       self._dscale)
+            self._scale_pending = True
+        self.t = ts[-1]
+        # final normalisation of rho and W by the last step's norm
+        self.plan.scale(self.wignerfunction, self._dscale)
+        self._flush_scale()
+        r = ring.get()
+        s0 = r[:, 0]
+        self.X_average.extend((r[:, 1] / s0).tolist())
+        self.X_average_RHS.extend((r[:, 2] / s0).tolist())
+        self.P_average.extend((r[:, 3] / s0).tolist())
+        self.P_average_RHS.extend((-r[:, 4] / s0).tolist())
+        self.hamiltonian_average.extend((r[:, 5] / s0).tolist())
+        return self.wignerfunction
+
+    def get_Ehrenfest(self, t):
+        """wigner_moyal_cuda_1d.py:584-606 (normalises, then appends the five averages)."""
+        if self.isEhrenfest:
+            self.normalize_rho_wigner()
+            self.X_average.append(self.get_average("X"))
+            self.X_average_RHS.append(self.get_average(self.diff_K))
+            self.P_average.append(self.get_average("P"))
+            self.P_average_RHS.append(-self.get_average(self.diff_V))
+            self.hamiltonian_average.append(self.get_average(self.hamiltonian))

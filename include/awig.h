@@ -1,0 +1,179 @@
+/*
+ * libawig: MI355X (gfx950) split-operator engine for phase-space propagators.
+ *
+ * C ABI: plain pointers and sizes, no torch types.  Every function returns an int status
+ * (AW_OK = 0); the message of the last failure on the calling thread is returned by
+ * aw_last_error().  Device pointers are HIP device addresses owned by the caller; the
+ * library owns plans (twiddle tables, partial-sum scratch) and compiled expression modules.
+ * A plan is used from one stream at a time (not re-entrant per plan, safe across plans).
+ *
+ * Arrays are C-order [batch][rows][cols] of complex (interleaved re, im), complex128
+ * (prec AW_PREC_F64) or complex64 (AW_PREC_F32).  "axis 0" FFTs run along rows index i
+ * (strided, one FFT per column), "axis 1" FFTs along the contiguous index j, exactly the
+ * two plan kinds of the reference FFT wrapper.
+ */
+#ifndef AWIG_H
+#define AWIG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes; the nonzero codes mirror the exception classes of cufft.py:31-104 */
+#define AW_OK 0
+#define AW_ERR_INVALID_PLAN 1
+#define AW_ERR_ALLOC_FAILED 2
+#define AW_ERR_INVALID_TYPE 3
+#define AW_ERR_INVALID_VALUE 4
+#define AW_ERR_INTERNAL 5
+#define AW_ERR_EXEC_FAILED 6
+#define AW_ERR_SETUP_FAILED 7
+#define AW_ERR_INVALID_SIZE 8
+#define AW_ERR_RTC 9
+#define AW_ERR_HIP 10
+
+#define AW_PREC_F64 0
+#define AW_PREC_F32 1
+
+#define AW_FORWARD (-1) /* exp(-2 pi i jk/N), unnormalised (cufft.py:6) */
+#define AW_INVERSE 1    /* exp(+2 pi i jk/N), unnormalised (cufft.py:7) */
+
+typedef struct aw_plan aw_plan;
+typedef struct aw_module aw_module;
+typedef void *aw_stream; /* hipStream_t; NULL = default stream */
+
+/* Library version (major*10000 + minor*100 + patch) and last error text. */
+int aw_version(void);
+const char *aw_last_error(void);
+
+/*
+ * Plan for a batch of [rows, cols] grids.  Replaces cufftPlanMany as called by
+ * Plan_Z2Z_2D_Axis0 / Plan_Z2Z_2D_Axis1 (cufft.py:274-345; the plan call itself at
+ * cufft.py:107-116): one plan serves both axes.  rows, cols: powers of two, 64..4096
+ * (cols may also be a single row of length 64..4096 with rows = 1).
+ */
+int aw_plan_create(aw_plan **plan, int rows, int cols, int batch, int prec);
+int aw_plan_destroy(aw_plan *plan); /* cufftDestroy, cufft.py:119-127 */
+
+/*
+ * In-place (or out-of-place when out != data) unnormalised batched FFT along one axis
+ * (axis 0 or 1), dir = AW_FORWARD / AW_INVERSE.  Replaces cufftExecZ2Z through
+ * fft_Z2Z / ifft_Z2Z (cufft.py:226-238).
+ */
+int aw_fft_axis(aw_plan *plan, void *data, void *out, int axis, int dir, aw_stream stream);
+
+/*
+ * One fused line pass over every line of the chosen axis of every batch item:
+ *     load -> [sign_in] -> [FFT dir_a] -> multiply by factor -> [FFT dir_b] -> [sign_out]
+ *          -> store (or store the diagonal element only)
+ * with factor(line l, element e) =
+ *     scale * dscale[b] * line_fac[b][l] * elem_fac[b][e] * mask[b][row, col]
+ *     * exp(i * chirp * lcoord(l) * ecoord(e)),
+ * lcoord(l) = lstep * (lpos(l) - lcenter), ecoord likewise, where lpos is l itself or its
+ * FFT-ordered index ((l + n/2) % n - n/2 + lcenter) when lfftorder is set.
+ * It replaces the reference's expV / expK / bloch_expV / bloch_expK / phase_shearX /
+ * phase_shearY / sign_flip / observable kernels together with the cuFFT calls around
+ * them, e.g. the sequence of single_step_propagation (wigner_moyal_cuda_1d.py:543-561).
+ * Vectors are complex in the plan's precision; mask is real in the plan's precision.
+ * NULL pointers mean "all ones".  out = NULL or data means in place.
+ */
+typedef struct aw_pass {
+    int axis;          /* 0: lines are columns (FFT along rows index i); 1: lines are rows */
+    int dir_a, dir_b;  /* 0 (none), AW_FORWARD or AW_INVERSE */
+    int sign_in, sign_out; /* 1: multiply by (-1)^(i+j) (0: no, 2: (-1)^i only) */
+    int out_mode;      /* 0: store the line; 1: store only element e == l into out[b][l] */
+    const void *line_fac; long long line_fac_bstride; /* elements between batch items */
+    const void *elem_fac; long long elem_fac_bstride;
+    const void *mask; long long mask_bstride;         /* real [rows][cols] */
+    double chirp, lstep, lcenter, estep, ecenter;
+    int lfftorder, efftorder;
+    double scale;
+    const double *dscale; /* complex (re, im) per batch item, or NULL */
+    void *out;
+} aw_pass;
+
+int aw_pass_run(aw_plan *plan, void *data, const aw_pass *pass, aw_stream stream);
+
+/*
+ * Build complex factor vectors in the plan's precision from FP64 inputs:
+ *   out[v][k] = scale * s(k) * mag[v][k]^mag_pow * E(c0 + ca * A[v][k] + cb * B[v][k])
+ * with E(x) = exp(i x) (mode 0) or exp(x) (mode 1), s(k) = (-1)^k if alt_sign else 1.
+ * nvec vectors of length n; A, B, mag may be NULL (zero / zero / one) and have strides
+ * a_stride, b_stride, mag_stride (0 broadcasts one vector to all nvec).  Phase arguments
+ * are summed and exponentiated in FP64 (the reference evaluates every phase in double,
+ * e.g. wigner_moyal_cuda_1d.py:751, 772); only the result is rounded to the plan precision.
+ */
+int aw_make_factors(aw_plan *plan, void *out, int n, int nvec, int mode,
+                    const double *A, long long a_stride, double ca,
+                    const double *B, long long b_stride, double cb,
+                    double c0, const double *mag, long long mag_stride, int mag_pow,
+                    double scale, int alt_sign, aw_stream stream);
+
+/*
+ * Reductions over each batch item of a [rows, cols] state (FP64 accumulation,
+ * deterministic order).  out[b][0..7] = { Re sum z, Im sum z, sum |z|^2, Re sum z^2,
+ * Im sum z^2, Re trace, Im trace, 0 } (trace = sum of z[k, k], square grids only).
+ * Replaces gpuarray.sum (wigner_moyal_cuda_1d.py:415, 617), gpuarray.dot without
+ * conjugation (:354), cu_linalg.trace (rho_vneumann_cuda_1d.py:383, 443) and the purity
+ * ReductionKernel (:574-579).  out is device memory (8 doubles per item).
+ */
+int aw_reduce_stats(aw_plan *plan, const void *data, double *out, aw_stream stream);
+
+/*
+ * out[b][c] = sum_i partials[b][i][c] for i < nblk, c < ncomp, summed in fixed order (the
+ * second stage of every block-partial reduction, including the hipRTC observable kernels).
+ */
+int aw_sum_partials(const double *partials, int nblk, int ncomp, int batch, double *out,
+                    aw_stream stream);
+
+/* Same statistics for an arbitrary complex vector of length n per item (for 1D states). */
+int aw_reduce_vec(int prec, const void *data, long long n, int batch, double *out,
+                  aw_stream stream);
+
+/*
+ * Scale factors from reductions (device-side, no host sync):
+ *   dscale[b] = 1 / ((mode 0: Re stats[b][comp]) or (mode 1: complex stats[b][comp..comp+1])
+ *               * factor)
+ * stats rows are stats_stride doubles apart.  Used for normalize_rho_wigner
+ * (wigner_moyal_cuda_1d.py:617-620) and trace renormalisation (rho_vneumann_cuda_1d.py:443,
+ * rho_bloch_cuda_1d.py:244).
+ */
+int aw_scale_from_stats(const double *stats, int stats_stride, int comp, int mode,
+                        double factor, double *dscale, int batch, aw_stream stream);
+
+/* data[b][k] *= dscale[b] (complex) * scale for n complex elements per item. */
+int aw_scale(int prec, void *data, long long n, int batch, const double *dscale,
+             double scale, aw_stream stream);
+
+/* Copy with precision conversion between complex128 and complex64 (n complex elements). */
+int aw_convert(const void *src, int src_prec, void *dst, int dst_prec, long long n,
+               aw_stream stream);
+
+/*
+ * Runtime compilation of generated HIP C source (the analogue of PyCUDA SourceModule,
+ * wigner_moyal_cuda_1d.py:241-258) for gfx950 with hipRTC, and generic launch of its
+ * extern "C" kernels.  log receives the compiler log (may be NULL).
+ */
+int aw_module_compile(aw_module **mod, const char *source, const char *name,
+                      char *log, size_t log_len);
+int aw_module_destroy(aw_module *mod);
+/* Compile only (no device needed): status, compiler log and code-object size. */
+int aw_rtc_check(const char *source, char *log, size_t log_len, size_t *code_size);
+int aw_module_launch(aw_module *mod, const char *kernel, unsigned gx, unsigned gy, unsigned gz,
+                     unsigned bx, unsigned by, unsigned bz, unsigned shmem, aw_stream stream,
+                     void **args);
+
+/* Free / total device memory in bytes (print_memory_info, wigner_moyal_cuda_1d.py:368-377). */
+int aw_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* hipDeviceSynchronize / hipStreamSynchronize with error capture. */
+int aw_sync(aw_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AWIG_H */

@@ -1,0 +1,134 @@
+"""GPU parity of the Wigner-Moyal path (libawig through the drop-in class) against the NumPy
+oracle restatement of wigner_moyal_cuda_1d.py."""
+import numpy as np
+import pytest
+
+from oracle.awig_oracle import WignerMoyalOracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(Nx=128, Np=128, Xa=10., Pa=10., dt=0.01, V="0.5 * X * X", Vf=lambda X, t: 0.5 * X * X,
+          ehrenfest=False, precision="fp64", **kw):
+    from accuratewigner_amd import WignerMoyalCUDA1D
+    extra = {}
+    oextra = {}
+    if ehrenfest:
+        extra = dict(diff_K="P", diff_V="X")
+        oextra = dict(diff_K=lambda X, P, t: P, diff_V=lambda X, P, t: X)
+    g = WignerMoyalCUDA1D(X_gridDIM=Nx, X_amplitude=Xa, P_gridDIM=Np, P_amplitude=Pa, dt=dt, t=0.,
+                          K="0.5 * P * P", V=V, precision=precision, **extra, **kw)
+    o = WignerMoyalOracle(X_gridDIM=Nx, X_amplitude=Xa, P_gridDIM=Np, P_amplitude=Pa, dt=dt,
+                          V=Vf, K=lambda P, t: 0.5 * P * P, **oextra)
+    return g, o
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("n", [64, 128, 256, 512, 1024, 2048])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_fft_axis_matches_numpy(n, prec):
+    import torch
+    from accuratewigner_amd.engine import Plan, FWD, INV
+    from accuratewigner_amd.device import to_device
+    rows, cols = (n, 128) if n != 128 else (128, 256)
+    batch = 2
+    rng = np.random.default_rng(n)
+    a = rng.normal(size=(batch, rows, cols)) + 1j * rng.normal(size=(batch, rows, cols))
+    dt = np.complex128 if prec == "fp64" else np.complex64
+    tol = 1e-13 if prec == "fp64" else 2e-6
+    for axis, d in [(0, FWD), (0, INV), (1, FWD), (1, INV)]:
+        p = Plan(rows, cols, batch, prec)
+        x = to_device(a.astype(dt))
+        p.fft(x, axis, d)
+        got = x.get()
+        ref = np.fft.fft(a, axis=axis + 1) if d == FWD else np.fft.ifft(a, axis=axis + 1, norm="forward")
+        assert rel(got, ref) < tol, (axis, d, rel(got, ref))
+    torch.cuda.synchronize()
+
+
+def test_wigner_transforms_match_oracle():
+    g, o = _pair(Nx=256, Np=128, Xa=8., Pa=6.)
+    g.set_wignerfunction("exp(-pow(X - 1., 2) - pow(P - 0.5, 2)) / M_PI")
+    o.set_wignerfunction(lambda X, P: np.exp(-(X - 1.) ** 2 - (P - 0.5) ** 2) / np.pi)
+    assert rel(g.wignerfunction.get(), o.wignerfunction) < 1e-13
+    assert rel(g.rho.get(), o.rho) < 1e-12
+    g.rho2wigner_()
+    o.rho2wigner_()
+    assert rel(g.wignerfunction.get(), o.wignerfunction) < 1e-12
+
+
+def test_wm_propagate_fp64_matches_oracle():
+    V = "0.5 * X * X + 0.1 * X * X * X * X"
+    g, o = _pair(V=V, Vf=lambda X, t: 0.5 * X * X + 0.1 * X ** 4)
+    g.set_wignerfunction("exp(-pow(X - 2., 2) - P * P) / M_PI")
+    o.set_wignerfunction(lambda X, P: np.exp(-(X - 2.) ** 2 - P * P) / np.pi)
+    W = g.propagate(300).get()
+    Wo = o.propagate(300)
+    assert rel(W, Wo) < 1e-10
+    assert rel(g.rho.get(), o.rho) < 1e-10
+    assert abs(g.t - o.t) == 0.0
+
+
+def test_wm_config1_full_run_fp64():
+    """Config 1: 256^2, x,p in [-10, 10), harmonic, W0 = exp(-(x-2)^2 - p^2)/pi, dt = 0.01,
+    1000 steps, norm/<x>/<p>/<H> every step; FP64 rel-L2 vs the oracle <= 1e-10."""
+    g, o = _pair(Nx=256, Np=256, ehrenfest=True)
+    g.set_wignerfunction("exp(-pow(X - 2., 2) - P * P) / M_PI")
+    o.set_wignerfunction(lambda X, P: np.exp(-(X - 2.) ** 2 - P * P) / np.pi)
+    W = g.propagate(1000).get()
+    Wo = o.propagate(1000)
+    assert rel(W, Wo) < 1e-10
+    for name in ("X_average", "P_average", "X_average_RHS", "P_average_RHS", "hamiltonian_average"):
+        a, b = np.array(getattr(g, name)), np.array(getattr(o, name))
+        assert a.shape == (1000,)
+        assert np.max(np.abs(a - b)) < 1e-10, name
+    t = 0.01 * np.arange(1, 1001)
+    assert np.max(np.abs(np.array(g.X_average) - 2 * np.cos(t))) < 1e-3   # exact rotation
+
+
+def test_wm_time_dependent_and_masks():
+    V = "0.5 * X * X + X * 0.1 * sin(2. * t)"
+    Vf = lambda X, t: 0.5 * X * X + X * 0.1 * np.sin(2. * t)  # noqa: E731
+    ax = "pow(abs(sin(0.5 * M_PI * (XX + XX_max) / XX_max) * sin(0.5 * M_PI * (XX_prime + XX_prime_max) / XX_prime_max)), dt * 0.05)"
+    g, o = _pair(V=V, Vf=Vf, abs_boundary_x=ax)
+    XXm, XXpm = g.XX.max(), g.XX_prime.max()
+    o.abs_boundary_x = lambda XX, XXp: np.abs(np.sin(0.5 * np.pi * (XX + XXm) / XXm)
+                                              * np.sin(0.5 * np.pi * (XXp + XXpm) / XXpm)) ** (0.01 * 0.05)
+    assert g._masks[0][0] == "vec"
+    g.set_wignerfunction("exp(-pow(X - 2., 2) - P * P) / M_PI")
+    o.set_wignerfunction(lambda X, P: np.exp(-(X - 2.) ** 2 - P * P) / np.pi)
+    assert rel(g.propagate(150).get(), o.propagate(150)) < 1e-10
+
+
+def test_wm_nonseparable_mask_table():
+    ap = "(abs(PP + PP_prime) < 5. ? 1. : 0.98)"
+    g, o = _pair(Nx=128, Np=64, abs_boundary_p=ap)
+    o.abs_boundary_p = lambda PP, PPp: np.where(np.abs(PP + PPp) < 5., 1., 0.98)
+    assert g._masks[1][0] == "table"
+    g.set_wignerfunction("exp(-pow(X - 2., 2) - P * P) / M_PI")
+    o.set_wignerfunction(lambda X, P: np.exp(-(X - 2.) ** 2 - P * P) / np.pi)
+    assert rel(g.propagate(100).get(), o.propagate(100)) < 1e-10
+
+
+def test_wm_observables_and_purity():
+    g, o = _pair()
+    g.set_wignerfunction("exp(-pow(X - 2., 2) - P * P) / M_PI")
+    o.set_wignerfunction(lambda X, P: np.exp(-(X - 2.) ** 2 - P * P) / np.pi)
+    g.propagate(50)
+    o.propagate(50)
+    assert abs(g.get_average("X * P + cos(X)") - o.get_average(lambda X, P, t: X * P + np.cos(X))) < 1e-12
+    assert abs(g.get_purity() - o.get_purity()) < 1e-12
+    assert abs(g.get_sigma_x_sigma_p() - o.get_sigma_x_sigma_p()) < 1e-11
+
+
+def test_wm_mixed_precision_error_budget():
+    g, o = _pair(precision="mixed")
+    g.set_wignerfunction("exp(-pow(X - 2., 2) - P * P) / M_PI")
+    o.set_wignerfunction(lambda X, P: np.exp(-(X - 2.) ** 2 - P * P) / np.pi)
+    W = g.propagate(200).get()
+    assert W.dtype == np.complex64
+    err = rel(W.astype(np.complex128), o.propagate(200))
+    assert err < 2e-5, err
