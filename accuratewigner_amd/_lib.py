@@ -108,6 +108,7 @@ _SIGNATURES = {
     "aw_module_launch": [_vp, ctypes.c_char_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
                          ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _vp,
                          ctypes.POINTER(_vp)],
+    "aw_project_out": [_i, _vp, _vp, _i, _i, _vp],
     "aw_mem_info": [ctypes.POINTER(_sz), ctypes.POINTER(_sz)],
     "aw_sync": [_vp],
 }

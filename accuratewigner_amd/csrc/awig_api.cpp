@@ -245,6 +245,13 @@ int aw_convert(const void *src, int src_prec, void *dst, int dst_prec, long long
     return launch_convert(src, src_prec, dst, dst_prec, n, (hipStream_t)stream);
 }
 
+int aw_project_out(int prec, void *psi, const void *phis, int nphi, int n, aw_stream stream) {
+    if (!psi || n < 1 || n > 65536 || nphi < 0 || (nphi > 0 && !phis))
+        return set_error(AW_ERR_INVALID_VALUE, "bad projection arguments");
+    if (nphi > 4096) return set_error(AW_ERR_INVALID_VALUE, "too many states to project out");
+    return launch_project_out(prec, psi, phis, nphi, n, (hipStream_t)stream);
+}
+
 int aw_mem_info(size_t *free_bytes, size_t *total_bytes) {
     AW_HIP(hipMemGetInfo(free_bytes, total_bytes));
     return AW_OK;

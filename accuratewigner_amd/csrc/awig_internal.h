@@ -55,6 +55,7 @@ int launch_scale_from_stats(const double *stats, int stride, int comp, int mode,
                             double *dscale, int batch, hipStream_t st);
 int launch_scale(int prec, void *data, long long n, int batch, const double *dscale, double scale,
                  hipStream_t st);
+int launch_project_out(int prec, void *psi, const void *phis, int nphi, int n, hipStream_t st);
 int launch_convert(const void *src, int sp, void *dst, int dp, long long n, hipStream_t st);
 
 }  // namespace awig

@@ -1,0 +1,337 @@
+// Fused FFT line-pass kernel core (templates shared by libawig and its tuning probes).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "awig_internal.h"
+
+namespace awig {
+
+// ---------------------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------------------
+template <typename R> struct Cx;
+template <> struct Cx<float> { using T = float2; };
+template <> struct Cx<double> { using T = double2; };
+
+template <typename T2> __device__ __forceinline__ T2 mk(double x, double y) {
+    T2 r;
+    r.x = static_cast<decltype(r.x)>(x);
+    r.y = static_cast<decltype(r.y)>(y);
+    return r;
+}
+template <typename T2> __device__ __forceinline__ T2 cadd(T2 a, T2 b) {
+    T2 r; r.x = a.x + b.x; r.y = a.y + b.y; return r;
+}
+template <typename T2> __device__ __forceinline__ T2 csub(T2 a, T2 b) {
+    T2 r; r.x = a.x - b.x; r.y = a.y - b.y; return r;
+}
+template <typename T2> __device__ __forceinline__ T2 cmul(T2 a, T2 b) {
+    T2 r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+template <typename T2, typename S> __device__ __forceinline__ T2 cscale(T2 a, S s) {
+    T2 r; r.x = a.x * s; r.y = a.y * s; return r;
+}
+
+// ---------------------------------------------------------------------------------------
+// compile-time roots of unity: cos/sin(2 pi k / n) by exact octant reduction of k/n and a
+// Taylor series on [0, pi/4] (evaluated by the compiler, never at run time)
+// ---------------------------------------------------------------------------------------
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+constexpr double taylor_sin(double x) {
+    double term = x, sum = x;
+    for (int n = 1; n < 16; ++n) {
+        term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double taylor_cos(double x) {
+    double term = 1.0, sum = 1.0;
+    for (int n = 1; n < 16; ++n) {
+        term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+        sum += term;
+    }
+    return sum;
+}
+struct CS { double c, s; };
+// (cos, sin)(2 pi a / n) for 0 <= a <= n/8 (n divisible by 8) or any a when n < 8
+constexpr CS root_small(long long a, long long n) {
+    return CS{taylor_cos(2.0 * kPi * double(a) / double(n)),
+              taylor_sin(2.0 * kPi * double(a) / double(n))};
+}
+constexpr CS root_cs(long long k, long long n) {
+    long long a = ((k % n) + n) % n;
+    bool neg_s = false;
+    if (2 * a > n) { a = n - a; neg_s = true; }           // theta -> 2pi - theta
+    bool neg_c = false;
+    if (4 * a > n) { a = n / 2 - a; neg_c = true; }        // theta -> pi - theta
+    CS r{1.0, 0.0};
+    if (8 * a > n && n % 8 == 0) {                         // theta -> pi/2 - theta
+        CS q = root_small(n / 4 - a, n);
+        r = CS{q.s, q.c};
+    } else {
+        r = root_small(a, n);
+    }
+    if (neg_c) r.c = -r.c;
+    if (neg_s) r.s = -r.s;
+    return r;
+}
+
+// multiply by exp(DIR * 2 pi i K / LEN), all arguments compile-time
+template <int DIR, int K, int LEN, typename T2>
+__device__ __forceinline__ T2 rot(T2 d) {
+    if constexpr (K == 0) {
+        return d;
+    } else if constexpr (4 * K == LEN) {
+        T2 r;
+        if constexpr (DIR < 0) { r.x = d.y; r.y = -d.x; }
+        else { r.x = -d.y; r.y = d.x; }
+        return r;
+    } else {
+        constexpr CS w = root_cs(K, LEN);
+        using R = decltype(d.x);
+        constexpr R c = static_cast<R>(w.c);
+        constexpr R s = static_cast<R>(DIR * w.s);
+        T2 r;
+        r.x = d.x * c - d.y * s;
+        r.y = d.x * s + d.y * c;
+        return r;
+    }
+}
+
+// in-register radix-2 decimation-in-frequency DFT of size N (natural order in and out)
+template <int DIR, int N, int LEN, int K, typename T2>
+__device__ __forceinline__ void dif_k(T2 *x) {
+    if constexpr (K < LEN / 2) {
+#pragma unroll
+        for (int s = 0; s < N; s += LEN) {
+            T2 a = x[s + K], b = x[s + K + LEN / 2];
+            x[s + K] = cadd(a, b);
+            x[s + K + LEN / 2] = rot<DIR, K, LEN>(csub(a, b));
+        }
+        dif_k<DIR, N, LEN, K + 1>(x);
+    }
+}
+template <int DIR, int N, int LEN, typename T2>
+__device__ __forceinline__ void dif_levels(T2 *x) {
+    if constexpr (LEN >= 2) {
+        dif_k<DIR, N, LEN, 0>(x);
+        dif_levels<DIR, N, LEN / 2>(x);
+    }
+}
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+constexpr int bitrev(int i, int bits) {
+    int r = 0;
+    for (int b = 0; b < bits; ++b) r |= ((i >> b) & 1) << (bits - 1 - b);
+    return r;
+}
+template <int DIR, int N, typename T2>
+__device__ __forceinline__ void dft_reg(T2 *x) {
+    if constexpr (N > 1) {
+        dif_levels<DIR, N, N>(x);
+        T2 y[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) y[i] = x[bitrev(i, ilog2(N))];
+#pragma unroll
+        for (int i = 0; i < N; ++i) x[i] = y[i];
+    }
+}
+
+// LDS layouts of the line buffers.
+//  rows (lanes of a wave walk one line): one region per line, one pad slot per 32 positions;
+//  columns (lanes interleave LPB adjacent lines): position-major [p][l] with one pad row per
+//  16 positions, so both the natural-order reads (u + T k) and the Stockham scatter writes
+//  (16 u + r) of a wave hit distinct banks.
+template <bool STRIDED, int N, int LPB> struct LdsLayout {
+    static constexpr int REGION = N + (N >> 5) + 1;
+    static constexpr int SIZE = STRIDED ? (N + (N >> 4)) * LPB : LPB * REGION;
+    static __device__ __forceinline__ int at(int p, int l) {
+        if constexpr (STRIDED) return (p + (p >> 4)) * LPB + l;
+        else return l * REGION + p + (p >> 5);
+    }
+};
+
+// Stockham stage with NS = product of the radices already applied.  Thread u owns the
+// elements u + T*k (k < E).  Butterfly b = u + T*m reads positions b + r*N/RAD, which are
+// the thread's own registers v[m + r*M]; outputs go to (b/NS)*NS*RAD + b%NS + r*NS.  In the
+// last stage those are again the thread's own registers, so no exchange follows it.
+template <typename T2, int N, int E, int NS, int DIR, typename LAY>
+__device__ __forceinline__ void stockham(T2 *v, T2 *L, int l, int u, const T2 *__restrict__ tw) {
+    constexpr int T = N / E;
+    constexpr int RAD = (N / NS < E) ? (N / NS) : E;
+    constexpr int M = E / RAD;
+    constexpr bool LAST = (NS * RAD == N);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int b = u + T * m;
+        const int k = b & (NS - 1);
+        T2 x[RAD];
+#pragma unroll
+        for (int r = 0; r < RAD; ++r) x[r] = v[m + r * M];
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int r = 1; r < RAD; ++r) {
+                T2 w = tw[(r * k) * (N / (NS * RAD))];
+                if constexpr (DIR > 0) w.y = -w.y;
+                x[r] = cmul(x[r], w);
+            }
+        }
+        dft_reg<DIR, RAD>(x);
+        if constexpr (LAST) {
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) v[m + r * M] = x[r];
+        } else {
+            const int base = (b - k) * RAD + k;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) L[LAY::at(base + r * NS, l)] = x[r];
+        }
+    }
+    if constexpr (!LAST) {
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < E; ++kk) v[kk] = L[LAY::at(u + T * kk, l)];
+        __syncthreads();
+        stockham<T2, N, E, NS * RAD, DIR, LAY>(v, L, l, u, tw);
+    }
+}
+
+template <typename T2, int N, int E, typename LAY>
+__device__ __forceinline__ void fft_line(int dir, T2 *v, T2 *L, int l, int u, const T2 *tw) {
+    if (dir < 0) stockham<T2, N, E, 1, -1, LAY>(v, L, l, u, tw);
+    else stockham<T2, N, E, 1, +1, LAY>(v, L, l, u, tw);
+}
+
+// ---------------------------------------------------------------------------------------
+// the fused line pass
+// ---------------------------------------------------------------------------------------
+// Compile-time configuration of one k_line_pass instantiation: precision R, line length N,
+// elements per thread E (threads per line T = N / E), lines per block LPB, line kind and
+// twiddle source (0: global table through the cache, 1: table copied into LDS).
+template <typename R_, int N_, int E_, int LPB_, bool STRIDED_, int TWM_> struct KCfg {
+    using R = R_;
+    static constexpr int N = N_, E = E_, LPB = LPB_, TWM = TWM_;
+    static constexpr bool STRIDED = STRIDED_;
+    static constexpr int T = N / E;
+    static constexpr int THREADS = LPB * T;
+    using LAY = LdsLayout<STRIDED, N, LPB>;
+    static constexpr int LDS_ELEMS = LAY::SIZE + (TWM == 1 ? N : 0);
+};
+
+template <typename C>
+__global__ __launch_bounds__(C::THREADS) void k_line_pass(PassArgs a) {
+    using R = typename C::R;
+    using T2 = typename Cx<R>::T;
+    using LAY = typename C::LAY;
+    constexpr int N = C::N, E = C::E, T = C::T, LPB = C::LPB;
+    constexpr bool STRIDED = C::STRIDED;
+    __shared__ T2 smem[C::LDS_ELEMS];
+
+    const int tid = threadIdx.x;
+    int l_loc, u;
+    if constexpr (STRIDED) { l_loc = tid % LPB; u = tid / LPB; }
+    else { l_loc = tid / T; u = tid % T; }
+    const long long gline = (long long)blockIdx.x * LPB + l_loc;
+    const bool active = gline < a.total_lines;
+    const long long b = active ? gline / a.nlines : 0;
+    const int line = active ? int(gline - b * a.nlines) : 0;
+    T2 *L = smem;
+    const T2 *tw = reinterpret_cast<const T2 *>(a.tw);
+    if constexpr (C::TWM == 1) {
+        // twiddle table into LDS; the first exchange barrier of any FFT precedes its first use
+        T2 *tl = smem + LAY::SIZE;
+        if (a.dir_a || a.dir_b)
+            for (int q = tid; q < N; q += C::THREADS) tl[q] = tw[q];
+        tw = tl;
+    }
+
+    const T2 *src = reinterpret_cast<const T2 *>(a.in) + b * a.bstride + (long long)line * a.line_stride;
+    T2 v[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int e = u + T * k;
+        v[k] = active ? src[(long long)e * a.elem_stride] : mk<T2>(0., 0.);
+    }
+    if (a.sign_in) {
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int e = u + T * k;
+            const int row = STRIDED ? e : line;
+            const int par = a.sign_in == 2 ? row : (line + e);
+            if (par & 1) v[k] = cscale(v[k], R(-1));
+        }
+    }
+    if (a.dir_a) fft_line<T2, N, E, LAY>(a.dir_a, v, L, l_loc, u, tw);
+
+    if (a.has_factor) {
+        // line-constant part: scale * dscale[b] * line_fac[b][line]
+        double2 lc = make_double2(a.scale, 0.0);
+        if (a.dscale) {
+            const double2 ds = reinterpret_cast<const double2 *>(a.dscale)[b];
+            lc = make_double2(lc.x * ds.x - lc.y * ds.y, lc.x * ds.y + lc.y * ds.x);
+        }
+        T2 lf = mk<T2>(lc.x, lc.y);
+        if (a.line_fac) {
+            const T2 q = reinterpret_cast<const T2 *>(a.line_fac)[b * a.lf_bstride + line];
+            lf = cmul(lf, q);
+        }
+        const T2 *ef = a.elem_fac ? reinterpret_cast<const T2 *>(a.elem_fac) + b * a.ef_bstride : nullptr;
+        const R *mk2 = a.mask ? reinterpret_cast<const R *>(a.mask) + b * a.m_bstride : nullptr;
+        double lcoord = 0.0;
+        if (a.chirp != 0.0) {
+            const int q = a.lfft ? ((line + a.nlines / 2) % a.nlines) : line;
+            lcoord = a.lstep * (double(q) - a.lcenter);
+        }
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int e = u + T * k;
+            T2 f = lf;
+            if (ef) f = cmul(f, ef[e]);
+            if (mk2) {
+                const long long row = STRIDED ? e : line, col = STRIDED ? line : e;
+                f = cscale(f, mk2[row * a.m_rowstride + col]);
+            }
+            if (a.chirp != 0.0) {
+                const int q = a.efft ? ((e + N / 2) % N) : e;
+                const double ph = a.chirp * (lcoord * (a.estep * (double(q) - a.ecenter)));
+                double s, c;
+                sincos(ph, &s, &c);
+                f = cmul(f, mk<T2>(c, s));
+            }
+            v[k] = cmul(v[k], f);
+        }
+    }
+
+    if (a.dir_b) fft_line<T2, N, E, LAY>(a.dir_b, v, L, l_loc, u, tw);
+    if (a.sign_out) {
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int e = u + T * k;
+            const int row = STRIDED ? e : line;
+            const int par = a.sign_out == 2 ? row : (line + e);
+            if (par & 1) v[k] = cscale(v[k], R(-1));
+        }
+    }
+    if (!active) return;
+    if (a.out_mode == 0) {
+        T2 *dst = reinterpret_cast<T2 *>(a.out) + b * a.bstride + (long long)line * a.line_stride;
+#pragma unroll
+        for (int k = 0; k < E; ++k) dst[(long long)(u + T * k) * a.elem_stride] = v[k];
+    } else {
+        // keep only the element on the diagonal (e == line) of this line
+        if ((line % T) == u && line < N) {
+            const int kk = line / T;
+            T2 val = v[0];
+#pragma unroll
+            for (int k = 0; k < E; ++k)
+                if (k == kk) val = v[k];
+            reinterpret_cast<T2 *>(a.out)[b * a.nlines + line] = val;
+        }
+    }
+}
+
+
+}  // namespace awig

@@ -1,0 +1,251 @@
+"""WignerBlochCUDA1D: drop-in MI355X implementation of the reference class
+(wigner_bloch_cuda_1d.py:9-366): the Boltzmann-Gibbs state and the ground state by
+imaginary-time (Bloch) split-operator propagation in the Wigner-Moyal representation.
+
+Bloch step (wigner_bloch_cuda_1d.py:221-240):  EVb, I0, F1, EKb, F0, I1, EVb, then
+normalize_rho_wigner, with EVb = S exp(-dbeta/4 [V(XX_j) + V(XX'_i) - V_min]) and
+EKb = exp(-dbeta/2 [K(PP_j) + K(PP'_i) - K_min]) at t_initial.  Both are separable, so
+they run as FP64-built real factor vectors inside the same fused passes as the real-time
+step: [EVb, I0] [F1, EKb, I1] [F0, EVb].
+"""
+import numpy as np
+
+from .device import current_stream, empty, to_device
+from .engine import FWD, INV
+from .wigner_moyal_cuda_1d import WignerMoyalCUDA1D
+
+CHECK_CHUNK = 32
+
+
+def _py2_round(x):
+    """Python 2 round() (half away from zero), as get_dbeta_num_beta_steps relies on."""
+    return float(np.sign(x) * np.floor(abs(x) + 0.5))
+
+
+class WignerBlochCUDA1D(WignerMoyalCUDA1D):
+    """Gibbs state rho = exp(-H/kT) of H = K(p) + V(x) by Bloch propagation in phase space
+    (reference: wigner_bloch_cuda_1d.py:9-18)."""
+
+    def __init__(self, **kwargs):
+        """Keywords of WignerMoyalCUDA1D plus kT, dbeta and t_initial
+        (wigner_bloch_cuda_1d.py:19-68)."""
+        kwargs = dict(kwargs)
+        if 't_initial' not in kwargs:
+            kwargs.update(t_initial=0.)
+            print("Warning: Initial time (t_initial) was not specified. So the default value was used t_initial = 0.")
+        self.kT = kwargs.pop('kT', 0.)
+        self.dbeta = kwargs.pop('dbeta', 0.01)
+        if 'dt' not in kwargs:
+            kwargs.update(dt=self.dbeta)
+        WignerMoyalCUDA1D.__init__(self, **kwargs)
+        self.V_min_x, self.V_min_xp, self.K_min_p, self.K_min_pp = self._separable_minima()
+        self.V_min = self.V_min_x + self.V_min_xp
+        self.K_min = self.K_min_p + self.K_min_pp
+        self.cuda_consts += "#define V_min %.15e \n" % self.V_min
+        self.cuda_consts += "#define K_min %.15e \n" % self.K_min
+        self._bloch_cache = {}
+        self.gibbs_stopped_at = None
+
+    def _eval_at_t0(self, which, xdev, n):
+        out = empty((1, n), np.float64)
+        ts = to_device(np.array([float(self.t_initial)]))
+        self._eval_1d(which, xdev, n, ts, 1, 0.0, out)
+        return out
+
+    def _separable_minima(self):
+        """fill_V / fill_K + gpuarray.min (wigner_bloch_cuda_1d.py:70-92, :343-365):
+        min_ij [V(XX_j) + V(XX'_i)] = min_j V(XX_j) + min_i V(XX'_i)."""
+        self._V0_x = self._eval_at_t0(0, self._dev_kXX, self.X_gridDIM)
+        self._V0_xp = self._eval_at_t0(0, self._dev_kXXp, self.P_gridDIM)
+        self._K0_p = self._eval_at_t0(1, self._dev_kPP, self.X_gridDIM)
+        self._K0_pp = self._eval_at_t0(1, self._dev_kPPp, self.P_gridDIM)
+        return (float(self._V0_x.get().min()), float(self._V0_xp.get().min()),
+                float(self._K0_p.get().min()), float(self._K0_pp.get().min()))
+
+    def get_V_min(self):
+        return self.V_min
+
+    def get_K_min(self):
+        return self.K_min
+
+    def _bloch_factors(self, dbeta):
+        """Real factor vectors of one Bloch step: [start col, end col, row] pass vectors."""
+        key = float(dbeta)
+        f = self._bloch_cache.get(key)
+        if f is not None:
+            return f
+        p = self.plan
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        colL = p.new_factors(Nx, 2)
+        colE = p.new_factors(Np, 2)
+        rowL = p.new_factors(Np, 1)
+        rowE = p.new_factors(Nx, 1)
+        q = -0.25 * dbeta
+        for v, scale in ((0, 1.0), (1, 1.0 / Np)):
+            p.factors(colL, Nx, 1, A=self._V0_x.ptr, ca=q, c0=-q * self.V_min_x, alt_sign=True,
+                      mode=1, out_offset=v)
+            p.factors(colE, Np, 1, A=self._V0_xp.ptr, ca=q, c0=-q * self.V_min_xp, alt_sign=True,
+                      mode=1, scale=scale, out_offset=v)
+        h = -0.5 * dbeta
+        p.factors(rowL, Np, 1, A=self._K0_pp.ptr, ca=h, c0=-h * self.K_min_pp, mode=1,
+                  scale=1.0 / Nx)
+        p.factors(rowE, Nx, 1, A=self._K0_p.ptr, ca=h, c0=-h * self.K_min_p, mode=1)
+        f = (colL, colE, rowL, rowE)
+        self._bloch_cache[key] = f
+        return f
+
+    def _bloch_step_passes(self, dbeta):
+        colL, colE, rowL, rowE = self._bloch_factors(dbeta)
+        csz = np.dtype(self.state_dtype).itemsize
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        p, rho = self.plan, self.rho
+        p.run(rho, 0, 0, INV, line_fac=colL.ptr, elem_fac=colE.ptr,
+              dscale=self._dscale if self._scale_pending else None)
+        self._scale_pending = False
+        p.run(rho, 1, FWD, INV, line_fac=rowL.ptr, elem_fac=rowE.ptr)
+        p.run(rho, 0, FWD, 0, line_fac=colL.ptr + Nx * csz, elem_fac=colE.ptr + Np * csz)
+
+    def bloch_single_step_propagation(self, dbeta):
+        """wigner_bloch_cuda_1d.py:221-240 (ends with normalize_rho_wigner)."""
+        self._bloch_step_passes(float(dbeta))
+        self.normalize_rho_wigner()
+
+    def _normalized_step(self, dbeta, stats_out):
+        """One Bloch step + Wigner transform; rho and W normalised lazily by a device-side
+        scale.  stats_out receives the 8 statistics of the unnormalised W."""
+        self._bloch_step_passes(dbeta)
+        self.rho2wigner_()
+        st = self.plan.stats(self.wignerfunction, out=stats_out)
+        self.plan.scale_from_stats(st.ptr, 8, 0, 0, self.dXdP, self._dscale)
+        self._scale_pending = True
+
+    def _finish_normalization(self):
+        if self._scale_pending:
+            self.plan.scale(self.wignerfunction, self._dscale)
+            self._flush_scale()
+
+    @classmethod
+    def get_dbeta_num_beta_steps(cls, kT, dbeta):
+        """wigner_bloch_cuda_1d.py:94-117."""
+        if kT > 0:
+            num_beta_steps = 1. / (kT * dbeta)
+            if _py2_round(num_beta_steps) != num_beta_steps:
+                num_beta_steps = _py2_round(num_beta_steps)
+                dbeta = 1. / (kT * num_beta_steps)
+            num_beta_steps = int(num_beta_steps)
+        else:
+            num_beta_steps = np.inf
+        return dbeta, num_beta_steps
+
+    def _purities(self, stats_host, n):
+        # purity = 2 pi Re sum(W^2) dXdP with W normalised by n = Re sum W dXdP (:349-354)
+        s = stats_host.reshape(n, self.batch, 8)
+        norm = s[:, :, 0] * self.dXdP
+        return 2. * np.pi * s[:, :, 3] * self.dXdP / norm ** 2
+
+    def get_gibbs_state(self, kT=None, dbeta=None, record=None):
+        """wigner_bloch_cuda_1d.py:119-155: infinite-temperature start, n Bloch steps with a
+        purity < 1 check after each (stop with a warning otherwise).  Steps run in chunks with
+        the checks read once per chunk; a violation replays the chunk up to that step.
+        ``record`` (not in the reference) may list C-expression observables whose averages
+        are recorded after every step into self.gibbs_record (shape [steps, n_obs])."""
+        if dbeta is None:
+            dbeta = self.dbeta
+        if kT is None:
+            kT = self.kT
+        dbeta, num_beta_steps = self.get_dbeta_num_beta_steps(kT, dbeta)
+        if not np.isfinite(num_beta_steps):
+            raise ValueError("kT = 0 gives an infinite number of beta steps (use get_ground_state)")
+        dbeta = np.float64(dbeta)
+        self.set_wignerfunction(1. / np.prod(self.wignerfunction.shape[-2:]))
+        self.gibbs_stopped_at = None
+        rec = []
+        k = 0
+        while k < num_beta_steps:
+            n = min(CHECK_CHUNK, num_beta_steps - k)
+            saved = self.rho.copy()
+            saved_scale = self._scale_pending
+            saved_ds = self._dscale.copy()
+            stats = empty((n, self.batch, 8), np.float64)
+            ring = empty((n, self.batch, len(record)), np.float64) if record else None
+            for s in range(n):
+                self._normalized_step(dbeta, stats[s])
+                if record:
+                    self._obs_launch(record, self.wignerfunction, float(self.t), ring.ptr + s * self.batch * len(record) * 8)
+            pur = self._purities(stats.get(), n)
+            bad = np.nonzero(~(pur < 1.))[0]
+            if len(bad):
+                stop = int(bad[0])
+                # replay up to and including the violating step (deterministic)
+                self.rho[:] = saved
+                self._scale_pending = saved_scale
+                self._dscale[:] = saved_ds
+                for s in range(stop + 1):
+                    self._normalized_step(dbeta, stats[s])
+                self._finish_normalization()
+                self.gibbs_stopped_at = k + stop
+                print("Warning: Gibbs state calculations interupted because purity = %.10f > 1."
+                      "Current kT = %.6f" % (float(np.max(pur[stop])), 1 / (max(k + stop, 1) * dbeta)))
+                if record:
+                    rec.append(self._record_values(ring.get()[:stop + 1], stats.get()[:stop + 1]))
+                break
+            if record:
+                rec.append(self._record_values(ring.get(), stats.get()))
+            k += n
+        self._finish_normalization()
+        if record:
+            self.gibbs_record = np.concatenate(rec, axis=0) if rec else np.zeros((0, len(record)))
+        return self.wignerfunction
+
+    def _record_values(self, ring, stats):
+        # average = sum Re W f / Re sum W (the dXdP of W's normalisation cancels)
+        vals = ring / stats[:, :, 0:1]
+        return vals[:, 0, :] if self.batch == 1 else vals
+
+    def get_ground_state(self, dbeta=None, abs_tol_purity=1e-12, max_steps=None):
+        """wigner_bloch_cuda_1d.py:157-219: adaptive imaginary-time descent; a step that
+        raises purity above 1, does not lower the energy or violates sigma_x sigma_p >= 1/2
+        is rolled back and dbeta halved.  Returns the unnormalised rho2wigner (:217)."""
+        if self.batch != 1:
+            raise NotImplementedError("get_ground_state is defined for a single state")
+        previous_energy = current_energy = np.inf
+        previous_purity = current_purity = 0.
+        if dbeta is None:
+            dbeta = self.dbeta
+        dbeta = np.float64(dbeta)
+        self.set_wignerfunction(1. / np.prod(self.wignerfunction.shape))
+        previous_rho = self.rho.copy()
+        exprs = [self.hamiltonian, "X * X", "X", "P * P", "P"]
+        stats = empty((1, 8), np.float64)
+        sums = empty((1, len(exprs)), np.float64)
+        nsteps = 0
+        while current_purity < (1. - abs_tol_purity) and dbeta > 1e-11:
+            if max_steps is not None and nsteps >= max_steps:
+                break
+            nsteps += 1
+            self.bloch_single_step_propagation(dbeta)
+            self.plan.stats(self.wignerfunction, out=stats)
+            self._obs_launch(exprs, self.wignerfunction, float(self.t), sums)
+            s = stats.get()[0]
+            o = sums.get()[0] * self.dXdP
+            current_purity = 2. * np.pi * s[3] * self.dXdP
+            ok = current_purity <= 1.
+            if ok:
+                current_energy = o[0]
+                ok = current_energy < previous_energy
+            if ok:
+                ok = np.sqrt((o[1] - o[2] ** 2) * (o[3] - o[4] ** 2)) >= 0.5
+            if ok:
+                previous_energy = current_energy
+                previous_purity = current_purity
+                previous_rho[:] = self.rho
+                print("Current energy: %.5f; purity: 1 - %.2e; dbeta: %.2e"
+                      % (current_energy, 1 - current_purity, dbeta))
+            else:
+                self.rho[:] = previous_rho
+                dbeta *= 0.5
+                current_energy = previous_energy
+                current_purity = previous_purity
+        self.ground_state_steps = nsteps
+        self.rho2wigner()
+        return self.wignerfunction

@@ -12,9 +12,10 @@ step reads and writes the state twice.  Phase factors are outer products of 1D v
 evaluated in FP64 from the user's C expressions (hipRTC), exactly the expV / expK phases
 -0.5 dt [V(XX_j) - V(XX'_i)] and -dt [K(PP_j) - K(PP'_i)] at t + dt/2 (:751, :772).
 
-Extra keyword (not in the reference): ``precision`` = "fp64" (default, complex128 state,
+Extra keywords (not in the reference): ``precision`` = "fp64" (default, complex128 state,
 the reference's arithmetic) or "mixed" (complex64 state and FFT arithmetic, FP64 phase
-arguments and FP64 reductions).
+arguments and FP64 reductions); ``batch`` = B > 1 propagates B independent states under the
+same Hamiltonian in every launch (arrays become [B, Np, Nx], averages become length-B arrays).
 """
 import ctypes
 from types import FunctionType, MethodType
@@ -42,6 +43,7 @@ class WignerMoyalCUDA1D:
         ``precision`` ("fp64" | "mixed")."""
         kwargs = dict(kwargs)
         self.precision = kwargs.pop("precision", "fp64")
+        self.batch = int(kwargs.pop("batch", 1))
         # save all attributes; Python functions become bound methods (:44-50)
         for name, value in kwargs.items():
             if isinstance(value, FunctionType):
@@ -93,16 +95,19 @@ class WignerMoyalCUDA1D:
         self.cuda_consts += getattr(self, "functions", "") or ""
 
         require_gpu()
-        self.plan = Plan(self.P_gridDIM, self.X_gridDIM, 1, self.precision)
+        self.plan = Plan(self.P_gridDIM, self.X_gridDIM, self.batch, self.precision)
         self.state_dtype = self.plan.state_dtype
         self._compile_physics()
 
         # state arrays (:268-271); W and rho in the chosen precision
         shape = (self.P_gridDIM, self.X_gridDIM)
+        if self.batch > 1:
+            shape = (self.batch,) + shape
         self.wignerfunction = DeviceArray(shape, self.state_dtype)
         self.rho = DeviceArray(shape, self.state_dtype)
-        self._dscale = empty((1, 2), np.float64)
+        self._dscale = empty((self.batch, 2), np.float64)
         self._scale_pending = False
+        self._pass_events = None   # list -> (start, end) torch events around each chunk's passes
 
         self.hamiltonian = self.K + ' + ' + self.V
         self.isEhrenfest = hasattr(self, "diff_K") and hasattr(self, "diff_V")
@@ -260,21 +265,25 @@ extern "C" __global__ void aw_obs(const %(st)s *W, long long n_item, double t, d
         mod = self._observable_kernel(exprs)
         n = self.X_gridDIM * self.P_gridDIM
         nblk = max(1, min(512, n // 2048))
-        partials = empty((nblk, len(exprs)), np.float64)
-        mod.launch("aw_obs", (nblk, 1), (256,), [W.ptr, ctypes.c_longlong(n), ctypes.c_double(t),
-                                                 partials.ptr], current_stream())
-        sum_partials(partials, nblk, len(exprs), 1, out)
+        partials = empty((self.batch, nblk, len(exprs)), np.float64)
+        mod.launch("aw_obs", (nblk, self.batch), (256,), [W.ptr, ctypes.c_longlong(n), ctypes.c_double(t),
+                                                          partials.ptr], current_stream())
+        sum_partials(partials, nblk, len(exprs), self.batch, out)
+
+    def _per_item(self, values):
+        values = np.asarray(values, dtype=np.float64)
+        return float(values[0]) if self.batch == 1 else values
 
     def get_average(self, observable):
         """Sum Re W * f(X, P, t) dXdP (wigner_moyal_cuda_1d.py:329-347)."""
-        out = empty((1,), np.float64)
+        out = empty((self.batch, 1), np.float64)
         self._obs_launch([observable], self.wignerfunction, float(self.t), out)
-        return float(out.get()[0]) * self.dXdP
+        return self._per_item(out.get()[:, 0] * self.dXdP)
 
     def get_purity(self):
         """2 pi Re dot(W, W) dXdP without conjugation (wigner_moyal_cuda_1d.py:349-354)."""
-        s = self.plan.stats(self.wignerfunction).get()[0]
-        return 2. * np.pi * s[3] * self.dXdP
+        s = self.plan.stats(self.wignerfunction).get()
+        return self._per_item(2. * np.pi * s[:, 3] * self.dXdP)
 
     def get_sigma_x_sigma_p(self):
         """wigner_moyal_cuda_1d.py:356-366."""
@@ -293,11 +302,12 @@ extern "C" __global__ void aw_obs(const %(st)s *W, long long n_item, double t, d
     # ------------------------------------------------------------------------------------
     def _init_kernel_source(self, expr):
         return (rtc.PRELUDE + self.cuda_consts + r"""
-extern "C" __global__ void aw_init(%(st)s *W, long long n_item)
+extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_total)
 {
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n_item;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n_total;
          idx += (long long)gridDim.x * blockDim.x) {
-        const size_t i = idx / X_gridDIM, j = idx %% X_gridDIM;
+        const long long loc = idx %% n_item;
+        const size_t i = loc / X_gridDIM, j = loc %% X_gridDIM;
         const double X = dX * (j - 0.5 * X_gridDIM);
         const double P = dP * (i - 0.5 * P_gridDIM);
         aw_state_store<%(st)s>::put(W + idx, cuda_complex(%(expr)s));
@@ -309,7 +319,8 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item)
         """wigner_moyal_cuda_1d.py:379-420: array, GPU array, Python function F(self, X, P),
         C string (X, P declared) or number; then W /= (complex) sum W dXdP and wigner2rho."""
         if isinstance(new_wigner_func, (np.ndarray, DeviceArray)):
-            assert new_wigner_func.shape == self.wignerfunction.shape, \
+            assert new_wigner_func.shape in (self.wignerfunction.shape,
+                                              self.wignerfunction.shape[-2:]), \
                 "The grid sizes does not match with the Wigner function"
             self.wignerfunction[:] = new_wigner_func
         elif isinstance(new_wigner_func, FunctionType):
@@ -318,8 +329,9 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item)
         elif isinstance(new_wigner_func, str):
             mod = rtc.Module(self._init_kernel_source(new_wigner_func), "wigner_moyal_init.hip")
             n = self.X_gridDIM * self.P_gridDIM
-            blocks, threads = rtc.launch_cfg(n)
-            mod.launch("aw_init", (blocks,), (threads,), [self.wignerfunction.ptr, ctypes.c_longlong(n)],
+            blocks, threads = rtc.launch_cfg(n * self.batch)
+            mod.launch("aw_init", (blocks,), (threads,), [self.wignerfunction.ptr, ctypes.c_longlong(n),
+                                                          ctypes.c_longlong(n * self.batch)],
                        current_stream())
         elif isinstance(new_wigner_func, (float, complex, int)):
             self.wignerfunction.fill(np.complex128(new_wigner_func))
@@ -476,6 +488,11 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item)
         _, _, tab2 = self._mask_args(0, 2)
         _, _, tabp = self._mask_args(1, 1)
         p, rho = self.plan, self.rho
+        ev = self._pass_events
+        if ev is not None:
+            import torch
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
         p.run(rho, 0, 0, INV, line_fac=colL.ptr, elem_fac=colE.ptr, mask=tab1, dscale=dscale)
         for s in range(n):
             p.run(rho, 1, FWD, INV, line_fac=rowL.ptr + s * Np * csz, elem_fac=rowE.ptr + s * Nx * csz,
@@ -483,6 +500,9 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item)
             last = s == n - 1
             p.run(rho, 0, FWD, 0 if last else INV, line_fac=colL.ptr + (s + 1) * Nx * csz,
                   elem_fac=colE.ptr + (s + 1) * Np * csz, mask=tab1 if last else tab2)
+        if ev is not None:
+            e1.record()
+            ev.append((e0, e1, 2 * n + 1))
 
     def single_step_propagation(self):
         """wigner_moyal_cuda_1d.py:543-561 (unnormalised result in self.rho)."""
@@ -509,26 +529,27 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item)
         # Ehrenfest: per step normalise through the Wigner transform and record
         # <X>, <diff_K>, <P>, -<diff_V>, <K + V> (:584-606) into a device ring; one host copy
         exprs = ["1.", "X", self.diff_K, "P", self.diff_V, self.hamiltonian]
-        ring = empty((steps, len(exprs)), np.float64)
+        nobs = len(exprs)
+        ring = empty((steps, self.batch, nobs), np.float64)
+        row_bytes = self.batch * nobs * 8
         for s in range(steps):
             self._run_chunk(ts[s:s + 1], dscale=self._dscale if self._scale_pending else None)
             self._scale_pending = False
             self.rho2wigner_()
-            self._obs_launch(exprs, self.wignerfunction, float(ts[s]), ring.ptr + s * len(exprs) * 8)
-            self.plan.scale_from_stats(ring.ptr + s * len(exprs) * 8, len(exprs), 0, 0, self.dXdP,
-                                       self._dscale)
+            self._obs_launch(exprs, self.wignerfunction, float(ts[s]), ring.ptr + s * row_bytes)
+            self.plan.scale_from_stats(ring.ptr + s * row_bytes, nobs, 0, 0, self.dXdP, self._dscale)
             self._scale_pending = True
         self.t = ts[-1]
         # final normalisation of rho and W by the last step's norm
         self.plan.scale(self.wignerfunction, self._dscale)
         self._flush_scale()
         r = ring.get()
-        s0 = r[:, 0]
-        self.X_average.extend((r[:, 1] / s0).tolist())
-        self.X_average_RHS.extend((r[:, 2] / s0).tolist())
-        self.P_average.extend((r[:, 3] / s0).tolist())
-        self.P_average_RHS.extend((-r[:, 4] / s0).tolist())
-        self.hamiltonian_average.extend((r[:, 5] / s0).tolist())
+        s0 = r[:, :, 0]
+        for lst, k, sgn in ((self.X_average, 1, 1.), (self.X_average_RHS, 2, 1.),
+                            (self.P_average, 3, 1.), (self.P_average_RHS, 4, -1.),
+                            (self.hamiltonian_average, 5, 1.)):
+            vals = sgn * r[:, :, k] / s0
+            lst.extend(vals[:, 0].tolist() if self.batch == 1 else list(vals))
         return self.wignerfunction
 
     def get_Ehrenfest(self, t):

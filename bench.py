@@ -1,0 +1,173 @@
+"""Throughput benchmark of the split-operator hot path on one MI355X.
+
+Workload (headline, mixed precision): Wigner-Moyal split-operator propagation of a batch of
+B independent 1024 x 1024 phase-space states (harmonic + quartic V, displaced Gaussian
+initial states), Strang steps dt = 0.01, ending with the reference's normalisation through
+the Wigner transform (propagate without Ehrenfest, wigner_moyal_cuda_1d.py:563-582).  The
+same states are also propagated by the plain FP64 path for the speedup and the error.
+
+A "step" is one Strang step of the whole batch.  value = B * 1024 * 1024 * steps / wall time
+(grid-point-steps per second), inputs resident in HBM.
+
+Prints ONE JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grid-point-steps/s vs plain FP64 at stated error; p50/p99 solve latency"
+HBM_PEAK_GBPS = 8000.0
+
+
+def make_system(N, batch, precision, L=10.0, dt=0.01):
+    from accuratewigner_amd import WignerMoyalCUDA1D
+    sysm = WignerMoyalCUDA1D(X_gridDIM=N, X_amplitude=L, P_gridDIM=N, P_amplitude=L, dt=dt, t=0.,
+                             K="0.5 * P * P", V="0.5 * X * X + 0.05 * X * X * X * X",
+                             precision=precision, batch=batch)
+    return sysm
+
+
+def initial_states(sysm, batch, seed=7):
+    rng = np.random.default_rng(seed)
+    x0 = rng.uniform(-2, 2, batch)
+    p0 = rng.uniform(-1, 1, batch)
+    X, P = sysm.X, sysm.P
+    W = np.exp(-(X[None] - x0[:, None, None]) ** 2 - (P[None] - p0[:, None, None]) ** 2) / np.pi
+    return W if batch > 1 else W[0]
+
+
+def time_propagate(sysm, warmup, steps):
+    import torch
+    sysm.propagate(warmup)
+    torch.cuda.synchronize()
+    sysm._pass_events = []
+    t0 = time.perf_counter()
+    sysm.propagate(steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    evs, sysm._pass_events = sysm._pass_events, None
+    pass_ms = sum(a.elapsed_time(b) for a, b, _ in evs)
+    npass = sum(n for _, _, n in evs)
+    return t1 - t0, pass_ms / max(npass, 1)
+
+
+def cpu_baseline(N, seconds, workers=None):
+    """Oracle (NumPy complex128) WM step on one 1024^2 state with precomputed factors."""
+    from oracle.awig_oracle import DFT, WignerMoyalOracle
+    o = WignerMoyalOracle(X_gridDIM=N, X_amplitude=10., P_gridDIM=N, P_amplitude=10., dt=0.01,
+                          V=lambda X, t: 0.5 * X * X + 0.05 * X ** 4, K=lambda P, t: 0.5 * P * P,
+                          dft=DFT(workers))
+    o.rho = np.exp(-(o.kXX[None, :] - 1.) ** 2 / 2 - (o.kXXp[:, None] - 1.) ** 2 / 2).astype(np.complex128)
+    o.propagate_precomputed(1)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.propagate_precomputed(1)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > seconds and n >= 2:
+            break
+    return N * N * n / el, n, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--grid", type=int, default=1024)
+    ap.add_argument("--batch", type=int, default=48)
+    ap.add_argument("--fp64-batch", type=int, default=16)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the propagators shard over independent states; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    N, B, K, W = args.grid, args.batch, args.steps, args.warmup
+
+    # -- headline: mixed precision, batch B -------------------------------------------------
+    mixed = make_system(N, B, "mixed")
+    W0 = initial_states(mixed, B)
+    mixed.set_wignerfunction(W0)
+    wall, pass_ms = time_propagate(mixed, W, K)
+    value = B * N * N * K / wall
+    bytes_per_pass = 2 * B * N * N * 8          # one read + one write of the c64 state
+    achieved = bytes_per_pass / (pass_ms * 1e-3) / 1e9
+
+    # -- plain FP64 path on the first Bf states ---------------------------------------------
+    Bf = min(args.fp64_batch, B)
+    fp64 = make_system(N, Bf, "fp64")
+    fp64.set_wignerfunction(W0[:Bf] if Bf > 1 else W0[0])
+    wall64, pass_ms64 = time_propagate(fp64, W, K)
+    value64 = Bf * N * N * K / wall64
+    a = mixed.wignerfunction.get()[:Bf].astype(np.complex128)
+    b = fp64.wignerfunction.get()
+    if Bf == 1:
+        b = b[None]
+    err = float(np.max(np.linalg.norm((a - b).reshape(Bf, -1), axis=1)
+                       / np.linalg.norm(b.reshape(Bf, -1), axis=1)))
+    del fp64
+
+    # -- solve latency: one 256^2 state, 1000 steps with Ehrenfest off (config-1 shape) -----
+    lat = []
+    one = make_system(256, 1, "mixed")
+    st = initial_states(one, 1)
+    for r in range(12):
+        one.set_wignerfunction(st)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        one.propagate(1000)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t0) * 1e3)
+    lat = np.array(lat[2:])
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_wm1024_mixed_b%d.json" % B)
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("hbm_bytes_per_pass")
+
+    cpu = None
+    if not args.no_cpu:
+        v1, n1, e1 = cpu_baseline(N, args.cpu_seconds)
+        cpu = {"value": v1, "unit": "gp-steps/s", "cores": 1, "kind": "port",
+               "sample": "oracle (NumPy complex128, numpy.fft) WM %dx%d single state, %d steps in %.1f s, "
+                         "time-independent factors precomputed" % (N, N, n1, e1),
+               "host_cpus": os.cpu_count()}
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "gp-steps/s", "n_gpus": 1, "steps": K,
+        "warmup": W, "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "Wigner-Moyal split-operator propagate (no Ehrenfest, normalised at end), "
+                               "%dx%d grid, batch of %d states, mixed precision (complex64 state, FP64 phases)"
+                               % (N, N, B),
+                   "grid": [N, N], "batch": B, "precision": "mixed", "dt": 0.01,
+                   "V": "0.5*X*X + 0.05*X^4", "K": "0.5*P*P"},
+        "fp64": {"value": value64, "unit": "gp-steps/s", "batch": Bf, "ms_per_step": wall64 / K * 1e3,
+                 "avg_pass_ms": pass_ms64,
+                 "achieved_GBps": 2 * Bf * N * N * 16 / (pass_ms64 * 1e-3) / 1e9},
+        "speedup_vs_fp64": value / value64,
+        "rel_l2_err_vs_fp64": err, "err_steps": W + K,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "kernel": "k_line_pass (row and column passes)", "avg_pass_ms": pass_ms,
+                     "bytes_per_pass": bytes_per_pass},
+        "latency": {"workload": "one 256x256 WM state, 1000 steps, mixed", "p50_ms": float(np.percentile(lat, 50)),
+                    "p99_ms": float(np.percentile(lat, 99))},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

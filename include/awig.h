@@ -166,6 +166,15 @@ int aw_module_launch(aw_module *mod, const char *kernel, unsigned gx, unsigned g
                      unsigned bx, unsigned by, unsigned bz, unsigned shmem, aw_stream stream,
                      void **args);
 
+/*
+ * Gram-Schmidt step of projectout_stationary_states (schrodinger_wigner_cuda_1d.py:505-550)
+ * on one vector of n complex elements (one workgroup, FP64 sums):
+ *   psi /= ||psi||;  c_k = sum conj(phi_k) psi;  psi -= sum_k c_k phi_k;  psi /= ||psi||.
+ * phis holds nphi vectors of length n contiguously (unit 2-norm, like the reference's
+ * stationary_states); n <= 65536.
+ */
+int aw_project_out(int prec, void *psi, const void *phis, int nphi, int n, aw_stream stream);
+
 /* Free / total device memory in bytes (print_memory_info, wigner_moyal_cuda_1d.py:368-377). */
 int aw_mem_info(size_t *free_bytes, size_t *total_bytes);
 

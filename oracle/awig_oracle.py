@@ -242,6 +242,22 @@ class WignerMoyalOracle:
         self.rho = EV * r
         return self.rho
 
+    def propagate_precomputed(self, steps):
+        """The single_step_propagation sequence (wigner_moyal_cuda_1d.py:543-561) with the
+        expV / expK factors computed once (valid for a time-independent H).  Used as the CPU
+        baseline timing loop; the result equals ``steps`` calls of single_step_propagation."""
+        F, I = self.dft.F, self.dft.I
+        EV = self.expV_factor(self.t) / np.sqrt(self.rho.shape[0] * self.rho.shape[1])
+        EK = self.expK_factor(self.t)
+        r = self.rho
+        for _ in range(steps):
+            r = F(I(EV * r, 0), 1)
+            r = I(F(EK * r, 0), 1)
+            r = EV * r
+            self.t += self.dt
+        self.rho = r
+        return self.rho
+
     def propagate(self, steps=1):
         """wigner_moyal_cuda_1d.py:563-582 (t advanced before each step, :571)."""
         for _ in range(steps):

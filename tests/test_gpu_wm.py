@@ -132,3 +132,23 @@ def test_wm_mixed_precision_error_budget():
     assert W.dtype == np.complex64
     err = rel(W.astype(np.complex128), o.propagate(200))
     assert err < 2e-5, err
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_wm_batch_equals_separate_runs(precision):
+    """B states in one launch give the same result as B separate single-state runs."""
+    from accuratewigner_amd import WignerMoyalCUDA1D
+    kw = dict(X_gridDIM=128, X_amplitude=8., P_gridDIM=64, P_amplitude=6., dt=0.01, t=0.,
+              K="0.5 * P * P", V="0.5 * X * X + 0.1 * X * X * X * X", precision=precision)
+    x0 = np.array([-1.0, 0.5, 2.0])
+    gb = WignerMoyalCUDA1D(batch=3, **kw)
+    X, P = gb.X, gb.P
+    W0 = np.exp(-(X[None] - x0[:, None, None]) ** 2 - P[None] ** 2) / np.pi
+    gb.set_wignerfunction(W0)
+    Wb = gb.propagate(40).get()
+    assert Wb.shape == (3, 64, 128)
+    for k in range(3):
+        g1 = WignerMoyalCUDA1D(**kw)
+        g1.set_wignerfunction(W0[k])
+        W1 = g1.propagate(40).get()
+        assert np.array_equal(Wb[k], W1), k

@@ -272,3 +272,12 @@ def test_sw_coherent_motion_and_wigner():
 def test_sign_matrix():
     S = sign_matrix(3, 4)
     assert S[0, 0] == 1 and S[1, 0] == -1 and S[0, 1] == -1 and S[2, 3] == -1
+
+
+def test_wm_precomputed_stepper_equals_reference_sequence():
+    a = _coherent_wm(ehrenfest=False, Nx=64, Np=64)
+    b = _coherent_wm(ehrenfest=False, Nx=64, Np=64)
+    for _ in range(10):
+        a.single_step_propagation()
+    b.propagate_precomputed(10)
+    assert np.linalg.norm(a.rho - b.rho) / np.linalg.norm(a.rho) < 1e-13
