@@ -40,50 +40,21 @@ int set_error(int code, const char *fmt, ...) {
 }
 
 // ---------------------------------------------------------------------------------------
-// chosen configurations per (precision, line length, line kind)
+// pass dispatch: plain factor passes run the specialised step kernels, the rest the general one
 // ---------------------------------------------------------------------------------------
-template <typename R, int N, bool STRIDED> struct PassCfg {
-    static constexpr int E = (N >= 256) ? 16 : 8;
-    static constexpr int T = N / E;
-    static constexpr int BYTES = sizeof(typename Cx<R>::T);
-    static constexpr int WANT = STRIDED ? ((sizeof(R) == 8 ? 4 : 8) > 256 / T
-                                               ? (sizeof(R) == 8 ? 4 : 8) : 256 / T)
-                                        : (256 / T > 1 ? 256 / T : 1);
-    static constexpr int FIT = (144 * 1024) / ((LdsLayout<STRIDED, N, 1>::SIZE + N) * BYTES);
-    static constexpr int LPB = WANT < FIT ? WANT : (FIT >= 1 ? FIT : 1);
-    using K = KCfg<R, N, E, LPB, STRIDED, 0>;
-};
-
-template <typename R, int N, bool STRIDED>
-static int launch_pass_n(const PassArgs &a, hipStream_t st) {
-    using C = typename PassCfg<R, N, STRIDED>::K;
-    const long long blocks = (a.total_lines + C::LPB - 1) / C::LPB;
-    if (blocks <= 0) return AW_OK;
-    if (blocks > 0x7fffffffLL) return set_error(AW_ERR_INVALID_SIZE, "too many lines (%lld)", a.total_lines);
-    hipLaunchKernelGGL((k_line_pass<C>), dim3((unsigned)blocks), dim3(C::THREADS), 0, st, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_line_pass launch: %s", hipGetErrorString(e));
-    return AW_OK;
-}
-
-template <typename R, bool STRIDED>
-static int launch_pass_r(int n, const PassArgs &a, hipStream_t st) {
-    switch (n) {
-    case 64: return launch_pass_n<R, 64, STRIDED>(a, st);
-    case 128: return launch_pass_n<R, 128, STRIDED>(a, st);
-    case 256: return launch_pass_n<R, 256, STRIDED>(a, st);
-    case 512: return launch_pass_n<R, 512, STRIDED>(a, st);
-    case 1024: return launch_pass_n<R, 1024, STRIDED>(a, st);
-    case 2048: return launch_pass_n<R, 2048, STRIDED>(a, st);
-    case 4096: return launch_pass_n<R, 4096, STRIDED>(a, st);
-    default: return set_error(AW_ERR_INVALID_SIZE, "unsupported line length %d (64..4096, power of two)", n);
-    }
-}
+int launch_pass_f32(int n, bool strided, int kind, const PassArgs &a, hipStream_t st);
+int launch_pass_f64(int n, bool strided, int kind, const PassArgs &a, hipStream_t st);
 
 int launch_pass(int prec, int n, bool strided, const PassArgs &a, hipStream_t st) {
-    if (prec == AW_PREC_F64)
-        return strided ? launch_pass_r<double, true>(n, a, st) : launch_pass_r<double, false>(n, a, st);
-    return strided ? launch_pass_r<float, true>(n, a, st) : launch_pass_r<float, false>(n, a, st);
+    int kind = 0;
+    const bool plain = a.chirp == 0.0 && !a.sign_in && !a.sign_out && a.out_mode == 0;
+    if (plain) {
+        if (a.dir_a == 0 && a.dir_b == AW_INVERSE) kind = 1;
+        else if (a.dir_a == AW_FORWARD && a.dir_b == AW_INVERSE) kind = 2;
+        else if (a.dir_a == AW_FORWARD && a.dir_b == 0) kind = 3;
+    }
+    return prec == AW_PREC_F64 ? launch_pass_f64(n, strided, kind, a, st)
+                               : launch_pass_f32(n, strided, kind, a, st);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -379,28 +350,6 @@ int launch_project_out(int prec, void *psi, const void *phis, int nphi, int n, h
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_project_out: %s", hipGetErrorString(e));
     return AW_OK;
-}
-
-int pass_lines_per_block(int prec, int n, bool strided) {
-#define AW_LPB(R, NN) \
-    (strided ? PassCfg<R, NN, true>::LPB : PassCfg<R, NN, false>::LPB)
-    if (prec == AW_PREC_F64) {
-        switch (n) {
-        case 64: return AW_LPB(double, 64); case 128: return AW_LPB(double, 128);
-        case 256: return AW_LPB(double, 256); case 512: return AW_LPB(double, 512);
-        case 1024: return AW_LPB(double, 1024); case 2048: return AW_LPB(double, 2048);
-        case 4096: return AW_LPB(double, 4096);
-        }
-    } else {
-        switch (n) {
-        case 64: return AW_LPB(float, 64); case 128: return AW_LPB(float, 128);
-        case 256: return AW_LPB(float, 256); case 512: return AW_LPB(float, 512);
-        case 1024: return AW_LPB(float, 1024); case 2048: return AW_LPB(float, 2048);
-        case 4096: return AW_LPB(float, 4096);
-        }
-    }
-#undef AW_LPB
-    return 0;
 }
 
 }  // namespace awig

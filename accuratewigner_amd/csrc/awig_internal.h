@@ -45,7 +45,6 @@ struct FactorArgs {
 };
 
 int launch_pass(int prec, int n, bool strided, const PassArgs &a, hipStream_t st);
-int pass_lines_per_block(int prec, int n, bool strided);
 int launch_make_factors(int prec, const FactorArgs &f, hipStream_t st);
 int launch_stats(int prec, const void *data, long long n_item, int rows, int cols, int batch,
                  double *partials, int nblk, double *out, hipStream_t st);

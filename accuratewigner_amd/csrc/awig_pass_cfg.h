@@ -1,0 +1,83 @@
+// Tuned k_line_pass configurations and their launchers (instantiated per precision in
+// awig_pass_f32.hip / awig_pass_f64.hip, so the two compile in parallel).
+#pragma once
+#include "awig_pass_core.h"
+
+namespace awig {
+
+constexpr int kLdsBudget = 160 * 1024;
+
+template <typename R, int N, bool STRIDED, int LPB, int TWM> constexpr int lds_bytes() {
+    return (LdsLayout<STRIDED, N, LPB>::SIZE + (TWM ? N : 0)) * int(sizeof(typename Cx<R>::T));
+}
+
+// Per (precision, line length, line kind):
+//  * E = 16 elements per thread (8 below 256), so a 1024-point line is 64 lanes, radix 16-16-4.
+//  * rows: 4 lines per 256-thread block in FP32, 2 lines per 128-thread block in FP64;
+//  * columns: 8 adjacent columns per block (64-byte row segments in FP32, 128 in FP64);
+//  * twiddles in LDS (FP32 always, FP64 columns when they fit) or read through the cache;
+//  * the largest power-of-two LPB that fits the 160 KB LDS.
+// Measured at 1024^2 x 48 (FP32) / x 24 (FP64) states on MI355X.
+template <typename R, int N, bool STRIDED> struct PassTune {
+    static constexpr int E = (N >= 256) ? 16 : 8;
+    static constexpr int T = N / E;
+    static constexpr bool F32 = sizeof(R) == 4;
+    static constexpr int WANT_ROWS = F32 ? (256 / T > 1 ? 256 / T : 1) : (128 / T > 1 ? 128 / T : 1);
+    static constexpr int WANT_COLS = (8 > 256 / T) ? 8 : 256 / T;
+    static constexpr int WANT0 = STRIDED ? WANT_COLS : WANT_ROWS;
+    // at most 512 threads per block (a 1024-thread block caps a thread at 128 VGPRs)
+    static constexpr int WANT = (WANT0 * T > 512) ? (512 / T > 1 ? 512 / T : 1) : WANT0;
+    static constexpr int TW_PREF = (F32 || STRIDED) ? 1 : 0;
+    static constexpr int fit(int lpb, int tw) {
+        return lpb <= 1 ? 1 : (lds_bytes_dyn(lpb, tw) <= kLdsBudget ? lpb : fit(lpb / 2, tw));
+    }
+    static constexpr int lds_bytes_dyn(int lpb, int tw) {
+        return ((STRIDED ? (N + (N >> 4)) * lpb : lpb * (N + (N >> 5) + 1)) + (tw ? N : 0)) *
+               int(sizeof(typename Cx<R>::T));
+    }
+    // keep LDS twiddles only if they do not cost lines per block
+    static constexpr int TWM = (TW_PREF && fit(WANT, 1) == fit(WANT, 0)) ? 1 : 0;
+    static constexpr int LPB = fit(WANT, TWM);
+};
+
+template <typename R, int N, bool STRIDED, int DA, int DB, int GEN>
+using PassK = KCfg<R, N, PassTune<R, N, STRIDED>::E, PassTune<R, N, STRIDED>::LPB, STRIDED,
+                   PassTune<R, N, STRIDED>::TWM, DA, DB, GEN>;
+
+template <typename C>
+int launch_cfg(const PassArgs &a, hipStream_t st) {
+    static_assert(lds_bytes<typename C::R, C::N, C::STRIDED, C::LPB, C::TWM>() <= kLdsBudget,
+                  "LDS budget");
+    const long long blocks = (a.total_lines + C::LPB - 1) / C::LPB;
+    if (blocks <= 0) return AW_OK;
+    if (blocks > 0x7fffffffLL) return set_error(AW_ERR_INVALID_SIZE, "too many lines (%lld)", a.total_lines);
+    hipLaunchKernelGGL((k_line_pass<C>), dim3((unsigned)blocks), dim3(C::THREADS), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_line_pass launch: %s", hipGetErrorString(e));
+    return AW_OK;
+}
+
+// kind: 0 general (run-time directions, signs, chirps, diagonal output);
+//       1 step [factor, inverse FFT]; 2 step [forward, factor, inverse]; 3 step [forward, factor]
+template <typename R, int N, bool STRIDED>
+int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
+    switch (kind) {
+    case 1: return launch_cfg<PassK<R, N, STRIDED, 0, 1, 0>>(a, st);
+    case 2: return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0>>(a, st);
+    case 3: return launch_cfg<PassK<R, N, STRIDED, -1, 0, 0>>(a, st);
+    default: return launch_cfg<PassK<R, N, STRIDED, 2, 2, 1>>(a, st);
+    }
+}
+
+template <typename R>
+int launch_pass_prec(int n, bool strided, int kind, const PassArgs &a, hipStream_t st) {
+#define AW_CASE(NN) \
+    case NN: return strided ? launch_kind_n<R, NN, true>(kind, a, st) : launch_kind_n<R, NN, false>(kind, a, st);
+    switch (n) {
+        AW_CASE(64) AW_CASE(128) AW_CASE(256) AW_CASE(512) AW_CASE(1024) AW_CASE(2048) AW_CASE(4096)
+    default: return set_error(AW_ERR_INVALID_SIZE, "unsupported line length %d (64..4096, power of two)", n);
+    }
+#undef AW_CASE
+}
+
+}  // namespace awig

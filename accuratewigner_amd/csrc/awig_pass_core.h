@@ -31,6 +31,13 @@ template <typename T2> __device__ __forceinline__ T2 cmul(T2 a, T2 b) {
     r.y = a.x * b.y + a.y * b.x;
     return r;
 }
+// a * conj(b)
+template <typename T2> __device__ __forceinline__ T2 cmul_conj(T2 a, T2 b) {
+    T2 r;
+    r.x = a.x * b.x + a.y * b.y;
+    r.y = a.y * b.x - a.x * b.y;
+    return r;
+}
 template <typename T2, typename S> __device__ __forceinline__ T2 cscale(T2 a, S s) {
     T2 r; r.x = a.x * s; r.y = a.y * s; return r;
 }
@@ -159,7 +166,11 @@ template <bool STRIDED, int N, int LPB> struct LdsLayout {
 // elements u + T*k (k < E).  Butterfly b = u + T*m reads positions b + r*N/RAD, which are
 // the thread's own registers v[m + r*M]; outputs go to (b/NS)*NS*RAD + b%NS + r*NS.  In the
 // last stage those are again the thread's own registers, so no exchange follows it.
-template <typename T2, int N, int E, int NS, int DIR, typename LAY>
+// TWM 0/1: tw is the flat table exp(-2 pi i t / N) (global or LDS).  TWM 2: tw is the
+// stage-major LDS table: for the stage of span NS and radix RAD, entry (NS - E) + k*(RAD-1)
+// + (r-1) holds exp(-2 pi i r k / (NS RAD)), so a butterfly reads its RAD-1 twiddles at
+// consecutive addresses from one base.
+template <typename T2, int N, int E, int NS, int DIR, typename LAY, int TWM = 0>
 __device__ __forceinline__ void stockham(T2 *v, T2 *L, int l, int u, const T2 *__restrict__ tw) {
     constexpr int T = N / E;
     constexpr int RAD = (N / NS < E) ? (N / NS) : E;
@@ -173,11 +184,12 @@ __device__ __forceinline__ void stockham(T2 *v, T2 *L, int l, int u, const T2 *_
 #pragma unroll
         for (int r = 0; r < RAD; ++r) x[r] = v[m + r * M];
         if constexpr (NS > 1) {
+            const T2 *tws = (TWM == 2) ? tw + (NS - E) + k * (RAD - 1) : tw;
 #pragma unroll
             for (int r = 1; r < RAD; ++r) {
-                T2 w = tw[(r * k) * (N / (NS * RAD))];
-                if constexpr (DIR > 0) w.y = -w.y;
-                x[r] = cmul(x[r], w);
+                const T2 w = (TWM == 2) ? tws[r - 1] : tw[(r * k) * (N / (NS * RAD))];
+                if constexpr (DIR > 0) x[r] = cmul_conj(x[r], w);
+                else x[r] = cmul(x[r], w);
             }
         }
         dft_reg<DIR, RAD>(x);
@@ -195,34 +207,42 @@ __device__ __forceinline__ void stockham(T2 *v, T2 *L, int l, int u, const T2 *_
 #pragma unroll
         for (int kk = 0; kk < E; ++kk) v[kk] = L[LAY::at(u + T * kk, l)];
         __syncthreads();
-        stockham<T2, N, E, NS * RAD, DIR, LAY>(v, L, l, u, tw);
+        stockham<T2, N, E, NS * RAD, DIR, LAY, TWM>(v, L, l, u, tw);
     }
 }
 
-template <typename T2, int N, int E, typename LAY>
+template <typename T2, int N, int E, typename LAY, int TWM>
 __device__ __forceinline__ void fft_line(int dir, T2 *v, T2 *L, int l, int u, const T2 *tw) {
-    if (dir < 0) stockham<T2, N, E, 1, -1, LAY>(v, L, l, u, tw);
-    else stockham<T2, N, E, 1, +1, LAY>(v, L, l, u, tw);
+    if (dir < 0) stockham<T2, N, E, 1, -1, LAY, TWM>(v, L, l, u, tw);
+    else stockham<T2, N, E, 1, +1, LAY, TWM>(v, L, l, u, tw);
 }
 
 // ---------------------------------------------------------------------------------------
 // the fused line pass
 // ---------------------------------------------------------------------------------------
 // Compile-time configuration of one k_line_pass instantiation: precision R, line length N,
-// elements per thread E (threads per line T = N / E), lines per block LPB, line kind and
-// twiddle source (0: global table through the cache, 1: table copied into LDS).
-template <typename R_, int N_, int E_, int LPB_, bool STRIDED_, int TWM_> struct KCfg {
+// elements per thread E (threads per line T = N / E), lines per block LPB, line kind,
+// twiddle source (0: global table through the cache, 1: table copied into LDS) and the FFT
+// directions: DA / DB in {-1, 0, +1} fixed at compile time, or 2 = chosen at run time.  A
+// "general" pass (GEN = 1) also handles sign flips, chirps and diagonal output; the step
+// passes (GEN = 0) only multiply by line x element factor vectors (and an optional mask),
+// which keeps their code to two FFT bodies.
+template <typename R_, int N_, int E_, int LPB_, bool STRIDED_, int TWM_, int DA_ = 2, int DB_ = 2,
+          int GEN_ = 1, int MINW_ = 1>
+struct KCfg {
     using R = R_;
-    static constexpr int N = N_, E = E_, LPB = LPB_, TWM = TWM_;
+    static constexpr int N = N_, E = E_, LPB = LPB_, TWM = TWM_, DA = DA_, DB = DB_, GEN = GEN_;
+    static constexpr int MINW = MINW_;   // minimum waves per SIMD the register budget must allow
     static constexpr bool STRIDED = STRIDED_;
     static constexpr int T = N / E;
     static constexpr int THREADS = LPB * T;
     using LAY = LdsLayout<STRIDED, N, LPB>;
-    static constexpr int LDS_ELEMS = LAY::SIZE + (TWM == 1 ? N : 0);
+    static constexpr int LDS_ELEMS = LAY::SIZE + (TWM != 0 ? N : 0);
 };
 
 template <typename C>
-__global__ __launch_bounds__(C::THREADS) void k_line_pass(PassArgs a) {
+__global__ __launch_bounds__(C::THREADS) __attribute__((amdgpu_waves_per_eu(C::MINW)))
+void k_line_pass(PassArgs a) {
     using R = typename C::R;
     using T2 = typename Cx<R>::T;
     using LAY = typename C::LAY;
@@ -234,28 +254,54 @@ __global__ __launch_bounds__(C::THREADS) void k_line_pass(PassArgs a) {
     int l_loc, u;
     if constexpr (STRIDED) { l_loc = tid % LPB; u = tid / LPB; }
     else { l_loc = tid / T; u = tid % T; }
-    const long long gline = (long long)blockIdx.x * LPB + l_loc;
+    // Column tiles that share 128-byte row segments run on the same XCD (blocks are dealt
+    // to the 8 XCDs round-robin), so the second half of each line hits that XCD's L2.
+    long long blk = blockIdx.x;
+    if constexpr (STRIDED) {
+        const long long nb = gridDim.x;
+        if ((nb & 7) == 0) blk = (blk & 7) * (nb >> 3) + (blk >> 3);
+    }
+    long long gline = blk * LPB + l_loc;
     const bool active = gline < a.total_lines;
-    const long long b = active ? gline / a.nlines : 0;
-    const int line = active ? int(gline - b * a.nlines) : 0;
+    if (!active) gline = a.total_lines - 1;   // compute on a valid line, store nothing
+    const int nl = a.nlines;
+    const long long b = gline / nl;
+    const int line = int(gline - b * nl);
     T2 *L = smem;
     const T2 *tw = reinterpret_cast<const T2 *>(a.tw);
-    if constexpr (C::TWM == 1) {
+    if constexpr (C::TWM != 0) {
         // twiddle table into LDS; the first exchange barrier of any FFT precedes its first use
         T2 *tl = smem + LAY::SIZE;
-        if (a.dir_a || a.dir_b)
-            for (int q = tid; q < N; q += C::THREADS) tl[q] = tw[q];
+        if ((C::DA == 2 ? a.dir_a != 0 : C::DA != 0) || (C::DB == 2 ? a.dir_b != 0 : C::DB != 0)) {
+            if constexpr (C::TWM == 1) {
+                for (int q = tid; q < N; q += C::THREADS) tl[q] = tw[q];
+            } else {
+                // stage-major layout: stage with span NS >= E owns entries [NS - E, NS*RAD - E)
+                for (int q = tid; q < N - E; q += C::THREADS) {
+                    int ns = E;
+                    while (q >= ns * ((N / ns < E) ? N / ns : E) - E) ns *= ((N / ns < E) ? N / ns : E);
+                    const int rad = (N / ns < E) ? N / ns : E;
+                    const int loc = q - (ns - E);
+                    const int k = loc / (rad - 1), r = loc % (rad - 1) + 1;
+                    tl[q] = tw[(r * k) * (N / (ns * rad))];
+                }
+            }
+        }
         tw = tl;
     }
 
-    const T2 *src = reinterpret_cast<const T2 *>(a.in) + b * a.bstride + (long long)line * a.line_stride;
+    // Addressing: an item is N x nl (columns) or nl x N (rows) elements; offsets inside an
+    // item fit 32 bits (<= 4096^2).  Element e = u + T k of the line sits at
+    //   rows: line * N + e          columns: e * nl + line
+    const long long item = b * (long long)N * nl;
+    const int off0 = STRIDED ? (u * nl + line) : (line * N + u);
+    const int kstep = STRIDED ? T * nl : T;
+    const T2 *src = reinterpret_cast<const T2 *>(a.in) + item + off0;
     T2 v[E];
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-        const int e = u + T * k;
-        v[k] = active ? src[(long long)e * a.elem_stride] : mk<T2>(0., 0.);
-    }
-    if (a.sign_in) {
+    for (int k = 0; k < E; ++k) v[k] = src[k * kstep];
+
+    if (C::GEN && a.sign_in) {
 #pragma unroll
         for (int k = 0; k < E; ++k) {
             const int e = u + T * k;
@@ -264,7 +310,11 @@ __global__ __launch_bounds__(C::THREADS) void k_line_pass(PassArgs a) {
             if (par & 1) v[k] = cscale(v[k], R(-1));
         }
     }
-    if (a.dir_a) fft_line<T2, N, E, LAY>(a.dir_a, v, L, l_loc, u, tw);
+    if constexpr (C::DA == 2) {
+        if (a.dir_a) fft_line<T2, N, E, LAY, C::TWM>(a.dir_a, v, L, l_loc, u, tw);
+    } else if constexpr (C::DA != 0) {
+        stockham<T2, N, E, 1, C::DA, LAY, C::TWM>(v, L, l_loc, u, tw);
+    }
 
     if (a.has_factor) {
         // line-constant part: scale * dscale[b] * line_fac[b][line]
@@ -278,35 +328,40 @@ __global__ __launch_bounds__(C::THREADS) void k_line_pass(PassArgs a) {
             const T2 q = reinterpret_cast<const T2 *>(a.line_fac)[b * a.lf_bstride + line];
             lf = cmul(lf, q);
         }
-        const T2 *ef = a.elem_fac ? reinterpret_cast<const T2 *>(a.elem_fac) + b * a.ef_bstride : nullptr;
-        const R *mk2 = a.mask ? reinterpret_cast<const R *>(a.mask) + b * a.m_bstride : nullptr;
-        double lcoord = 0.0;
-        if (a.chirp != 0.0) {
-            const int q = a.lfft ? ((line + a.nlines / 2) % a.nlines) : line;
-            lcoord = a.lstep * (double(q) - a.lcenter);
-        }
+        if (a.elem_fac) {
+            const T2 *ef = reinterpret_cast<const T2 *>(a.elem_fac) + b * a.ef_bstride + u;
 #pragma unroll
-        for (int k = 0; k < E; ++k) {
-            const int e = u + T * k;
-            T2 f = lf;
-            if (ef) f = cmul(f, ef[e]);
-            if (mk2) {
-                const long long row = STRIDED ? e : line, col = STRIDED ? line : e;
-                f = cscale(f, mk2[row * a.m_rowstride + col]);
-            }
-            if (a.chirp != 0.0) {
+            for (int k = 0; k < E; ++k) v[k] = cmul(v[k], cmul(lf, ef[T * k]));
+        } else {
+#pragma unroll
+            for (int k = 0; k < E; ++k) v[k] = cmul(v[k], lf);
+        }
+        if (a.mask) {
+            const R *mk2 = reinterpret_cast<const R *>(a.mask) + item + off0;
+#pragma unroll
+            for (int k = 0; k < E; ++k) v[k] = cscale(v[k], mk2[k * kstep]);
+        }
+        if (C::GEN && a.chirp != 0.0) {
+            const int ql = a.lfft ? ((line + nl / 2) % nl) : line;
+            const double lcoord = a.lstep * (double(ql) - a.lcenter);
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const int e = u + T * k;
                 const int q = a.efft ? ((e + N / 2) % N) : e;
                 const double ph = a.chirp * (lcoord * (a.estep * (double(q) - a.ecenter)));
-                double s, c;
-                sincos(ph, &s, &c);
-                f = cmul(f, mk<T2>(c, s));
+                double sn, cs;
+                sincos(ph, &sn, &cs);
+                v[k] = cmul(v[k], mk<T2>(cs, sn));
             }
-            v[k] = cmul(v[k], f);
         }
     }
 
-    if (a.dir_b) fft_line<T2, N, E, LAY>(a.dir_b, v, L, l_loc, u, tw);
-    if (a.sign_out) {
+    if constexpr (C::DB == 2) {
+        if (a.dir_b) fft_line<T2, N, E, LAY, C::TWM>(a.dir_b, v, L, l_loc, u, tw);
+    } else if constexpr (C::DB != 0) {
+        stockham<T2, N, E, 1, C::DB, LAY, C::TWM>(v, L, l_loc, u, tw);
+    }
+    if (C::GEN && a.sign_out) {
 #pragma unroll
         for (int k = 0; k < E; ++k) {
             const int e = u + T * k;
@@ -316,10 +371,10 @@ __global__ __launch_bounds__(C::THREADS) void k_line_pass(PassArgs a) {
         }
     }
     if (!active) return;
-    if (a.out_mode == 0) {
-        T2 *dst = reinterpret_cast<T2 *>(a.out) + b * a.bstride + (long long)line * a.line_stride;
+    if (!C::GEN || a.out_mode == 0) {
+        T2 *dst = reinterpret_cast<T2 *>(a.out) + item + off0;
 #pragma unroll
-        for (int k = 0; k < E; ++k) dst[(long long)(u + T * k) * a.elem_stride] = v[k];
+        for (int k = 0; k < E; ++k) dst[k * kstep] = v[k];
     } else {
         // keep only the element on the diagonal (e == line) of this line
         if ((line % T) == u && line < N) {
@@ -328,10 +383,9 @@ __global__ __launch_bounds__(C::THREADS) void k_line_pass(PassArgs a) {
 #pragma unroll
             for (int k = 0; k < E; ++k)
                 if (k == kk) val = v[k];
-            reinterpret_cast<T2 *>(a.out)[b * a.nlines + line] = val;
+            reinterpret_cast<T2 *>(a.out)[b * nl + line] = val;
         }
     }
 }
-
 
 }  // namespace awig

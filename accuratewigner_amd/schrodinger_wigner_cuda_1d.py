@@ -285,8 +285,11 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
             ts[k] = t
         return ts
 
-    def _factors(self, ts, bloch=False):
-        """Per-step EV (with the merged pair for consecutive steps) and EK vectors."""
+    def _factors(self, ts, bloch=False, raw=False):
+        """Per-step EV (with the merged pair for consecutive steps) and EK vectors.  Unless
+        ``raw``, EK carries the 1/N of the F-I pair: the reference has no 1/N (:356-369) but
+        renormalises every step (:388), so the scale cancels and the deferred normalisation
+        of a chunk cannot overflow."""
         n = len(ts)
         N, dt, p = self.X_gridDIM, self.dt, self.plan
         ts_dev = to_device(ts)
@@ -305,17 +308,17 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
             p.factors(ev, N, n - 1, A=Vt.ptr, ca=cv, B=Vt.ptr + N * 8, cb=cv, mag=bx, mag_pow=2,
                       mode=mode, out_offset=1)
         p.factors(ev, N, 1, A=Vt.ptr + (n - 1) * N * 8, ca=cv, mag=bx, mode=mode, out_offset=n)
-        p.factors(ek, N, n, A=Kt.ptr, ca=-dt, mag=bp, mode=mode)
+        p.factors(ek, N, n, A=Kt.ptr, ca=-dt, mag=bp, mode=mode, scale=1.0 if raw else 1.0 / N)
         return ev, ek
 
-    def _run(self, ts, bloch=False, per_step=None):
+    def _run(self, ts, bloch=False, per_step=None, raw=False):
         """Steps at times ts; merged half-kicks unless ``per_step`` needs psi after each step."""
         n = len(ts)
         N = self.X_gridDIM
         csz = np.dtype(self.state_dtype).itemsize
         p, psi = self.plan, self.wavefunction
         if per_step is None:
-            ev, ek = self._factors(ts, bloch)
+            ev, ek = self._factors(ts, bloch, raw)
             p.run(psi, 1, 0, FWD, elem_fac=ev.ptr, dscale=self._dscale if self._scale_pending else None)
             self._scale_pending = False
             for s in range(n):
@@ -335,7 +338,7 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
     def single_step_propagation(self):
         """:356-369 (unnormalised)."""
         self._flush_scale()
-        self._run(np.array([self.t], np.float64))
+        self._run(np.array([self.t], np.float64), raw=True)
         return self.wavefunction
 
     def propagate(self, steps=1):
@@ -433,7 +436,7 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
         self._phis = DeviceArray((max(nstates, 1), N), self.state_dtype)
         even = True
         t0 = np.full(1, float(self.t_initial))
-        ev, ek = self._factors(t0, bloch=True)
+        ev, ek = self._factors(t0, bloch=True, raw=True)
         csz = np.dtype(self.state_dtype).itemsize
         p, psi = self.plan, self.wavefunction
         for n in range(nstates):

@@ -82,8 +82,8 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--grid", type=int, default=1024)
-    ap.add_argument("--batch", type=int, default=48)
-    ap.add_argument("--fp64-batch", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=96)
+    ap.add_argument("--fp64-batch", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
