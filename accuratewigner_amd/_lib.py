@@ -84,6 +84,20 @@ class aw_pass(ctypes.Structure):
     ]
 
 
+class aw_steps(ctypes.Structure):
+    """Mirror of ``struct aw_steps`` in include/awig.h."""
+    _fields_ = [
+        ("nsteps", ctypes.c_int), ("const_steps", ctypes.c_int), ("batch", ctypes.c_int),
+        ("col_line", ctypes.c_void_p), ("col_elem", ctypes.c_void_p),
+        ("row_line", ctypes.c_void_p), ("row_elem", ctypes.c_void_p),
+        ("col_line_bstride", ctypes.c_longlong), ("col_elem_bstride", ctypes.c_longlong),
+        ("row_line_bstride", ctypes.c_longlong), ("row_elem_bstride", ctypes.c_longlong),
+        ("mask_open_close", ctypes.c_void_p), ("mask_mid", ctypes.c_void_p), ("mask_row", ctypes.c_void_p),
+        ("mask_bstride", ctypes.c_longlong),
+        ("dscale", ctypes.c_void_p),
+    ]
+
+
 _vp, _i, _ll, _d, _sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_size_t
 
 # name -> argtypes (all functions return int status except noted)
@@ -93,6 +107,7 @@ _SIGNATURES = {
     "aw_plan_destroy": [_vp],
     "aw_fft_axis": [_vp, _vp, _vp, _i, _i, _vp],
     "aw_pass_run": [_vp, _vp, ctypes.POINTER(aw_pass), _vp],
+    "aw_step_sequence": [_vp, _vp, ctypes.POINTER(aw_steps), _vp],
     "aw_make_factors": [_vp, _vp, _i, _i, _i, _vp, _ll, _d, _vp, _ll, _d, _d, _vp, _ll, _i,
                         _d, _i, _vp],
     "aw_reduce_stats": [_vp, _vp, _vp, _vp],

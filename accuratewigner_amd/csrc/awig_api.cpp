@@ -166,6 +166,52 @@ int aw_pass_run(aw_plan *p, void *data, const aw_pass *ps, aw_stream stream) {
     return launch_pass(p->prec, n, strided, a, (hipStream_t)stream);
 }
 
+int aw_step_sequence(aw_plan *p, void *data, const aw_steps *st, aw_stream stream) {
+    if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
+    if (!data || !st) return set_error(AW_ERR_INVALID_VALUE, "NULL data or steps");
+    if (st->nsteps < 1) return set_error(AW_ERR_INVALID_VALUE, "nsteps must be >= 1");
+    if (p->rows < 2) return set_error(AW_ERR_INVALID_VALUE, "step sequences need a 2D plan");
+    if (st->batch < 0 || st->batch > p->batch)
+        return set_error(AW_ERR_INVALID_VALUE, "batch %d outside 0..%d", st->batch, p->batch);
+    const long long esz = p->csize;
+    const char *cl = static_cast<const char *>(st->col_line), *ce = static_cast<const char *>(st->col_elem);
+    const char *rl = static_cast<const char *>(st->row_line), *re = static_cast<const char *>(st->row_elem);
+    if (!cl || !ce || !rl || !re) return set_error(AW_ERR_INVALID_VALUE, "NULL factor table");
+    const int n = st->nsteps;
+    const int saved_batch = p->batch;
+    if (st->batch > 0) p->batch = st->batch;   // run the active prefix only
+    // column vector v of the sequence and row vector s (constant tables reuse 0/1/2 and 0)
+    auto cvec = [&](int v) { return st->const_steps ? (v == 0 ? 0 : (v == n ? 2 : 1)) : v; };
+    auto rvec = [&](int s) { return st->const_steps ? 0 : s; };
+    aw_pass ps{};
+    ps.scale = 1.0;
+    ps.mask_bstride = st->mask_bstride;
+    // opening column pass: factor c_0 then inverse FFT along axis 0
+    ps.axis = 0; ps.dir_a = 0; ps.dir_b = AW_INVERSE;
+    ps.line_fac = cl; ps.line_fac_bstride = st->col_line_bstride;
+    ps.elem_fac = ce; ps.elem_fac_bstride = st->col_elem_bstride;
+    ps.mask = st->mask_open_close;
+    ps.dscale = st->dscale;
+    int rc = aw_pass_run(p, data, &ps, stream);
+    ps.dscale = nullptr;
+    for (int s = 0; rc == AW_OK && s < n; ++s) {
+        ps.axis = 1; ps.dir_a = AW_FORWARD; ps.dir_b = AW_INVERSE;
+        ps.line_fac = rl + rvec(s) * p->rows * esz; ps.line_fac_bstride = st->row_line_bstride;
+        ps.elem_fac = re + rvec(s) * p->cols * esz; ps.elem_fac_bstride = st->row_elem_bstride;
+        ps.mask = st->mask_row;
+        rc = aw_pass_run(p, data, &ps, stream);
+        if (rc != AW_OK) break;
+        const bool last = s == n - 1;
+        ps.axis = 0; ps.dir_a = AW_FORWARD; ps.dir_b = last ? 0 : AW_INVERSE;
+        ps.line_fac = cl + cvec(s + 1) * p->cols * esz; ps.line_fac_bstride = st->col_line_bstride;
+        ps.elem_fac = ce + cvec(s + 1) * p->rows * esz; ps.elem_fac_bstride = st->col_elem_bstride;
+        ps.mask = last ? st->mask_open_close : st->mask_mid;
+        rc = aw_pass_run(p, data, &ps, stream);
+    }
+    p->batch = saved_batch;
+    return rc;
+}
+
 int aw_fft_axis(aw_plan *p, void *data, void *out, int axis, int dir, aw_stream stream) {
     if (dir != AW_FORWARD && dir != AW_INVERSE)
         return set_error(AW_ERR_INVALID_VALUE, "direction must be -1 or +1 (got %d)", dir);

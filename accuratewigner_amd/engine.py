@@ -70,6 +70,25 @@ class Plan:
         _lib.call("aw_pass_run", self.handle, ctypes.c_void_p(_p(data)), ctypes.byref(ps),
                   ctypes.c_void_p(stream if stream is not None else current_stream()))
 
+    def steps(self, data, nsteps, col_line, col_elem, row_line, row_elem, col_line_bstride=0,
+              col_elem_bstride=0, row_line_bstride=0, row_elem_bstride=0, mask_open_close=None,
+              mask_mid=None, mask_row=None, mask_bstride=0, dscale=None, stream=None,
+              const_steps=False, batch=0):
+        """n merged Strang steps (2n + 1 passes) in one native call (aw_step_sequence)."""
+        st = _lib.aw_steps()
+        st.nsteps = nsteps
+        st.const_steps = int(bool(const_steps))
+        st.batch = int(batch)
+        st.col_line, st.col_elem = _p(col_line), _p(col_elem)
+        st.row_line, st.row_elem = _p(row_line), _p(row_elem)
+        st.col_line_bstride, st.col_elem_bstride = col_line_bstride, col_elem_bstride
+        st.row_line_bstride, st.row_elem_bstride = row_line_bstride, row_elem_bstride
+        st.mask_open_close, st.mask_mid, st.mask_row = _p(mask_open_close), _p(mask_mid), _p(mask_row)
+        st.mask_bstride = mask_bstride
+        st.dscale = _p(dscale)
+        _lib.call("aw_step_sequence", self.handle, ctypes.c_void_p(_p(data)), ctypes.byref(st),
+                  ctypes.c_void_p(stream if stream is not None else current_stream()))
+
     def fft(self, data, axis, direction, out=None):
         _lib.call("aw_fft_axis", self.handle, ctypes.c_void_p(_p(data)),
                   ctypes.c_void_p(_p(out)), axis, direction, ctypes.c_void_p(current_stream()))

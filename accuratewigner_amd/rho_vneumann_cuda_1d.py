@@ -284,12 +284,7 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
             import torch
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
-        p.run(rho, 0, 0, INV, line_fac=colL.ptr, elem_fac=colE.ptr, dscale=dscale)
-        for s in range(n):
-            p.run(rho, 1, FWD, INV, line_fac=rowL.ptr + s * N * csz, elem_fac=rowE.ptr + s * N * csz)
-            last = s == n - 1
-            p.run(rho, 0, FWD, 0 if last else INV, line_fac=colL.ptr + (s + 1) * N * csz,
-                  elem_fac=colE.ptr + (s + 1) * N * csz)
+        p.steps(rho, n, colL, colE, rowL, rowE, dscale=dscale)
         if ev is not None:
             e1.record()
             ev.append((e0, e1, 2 * n + 1))

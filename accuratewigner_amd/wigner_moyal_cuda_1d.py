@@ -493,13 +493,8 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             import torch
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
-        p.run(rho, 0, 0, INV, line_fac=colL.ptr, elem_fac=colE.ptr, mask=tab1, dscale=dscale)
-        for s in range(n):
-            p.run(rho, 1, FWD, INV, line_fac=rowL.ptr + s * Np * csz, elem_fac=rowE.ptr + s * Nx * csz,
-                  mask=tabp)
-            last = s == n - 1
-            p.run(rho, 0, FWD, 0 if last else INV, line_fac=colL.ptr + (s + 1) * Nx * csz,
-                  elem_fac=colE.ptr + (s + 1) * Np * csz, mask=tab1 if last else tab2)
+        p.steps(rho, n, colL, colE, rowL, rowE, mask_open_close=tab1, mask_mid=tab2, mask_row=tabp,
+                dscale=dscale)
         if ev is not None:
             e1.record()
             ev.append((e0, e1, 2 * n + 1))

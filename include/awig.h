@@ -98,6 +98,36 @@ typedef struct aw_pass {
 int aw_pass_run(aw_plan *plan, void *data, const aw_pass *pass, aw_stream stream);
 
 /*
+ * n merged Strang steps of a 2D split-operator propagator in 2n + 1 fused passes:
+ *   [col: factor c_0 -> I0]  then for s < n:  [row: F1 -> factor r_s -> I1]
+ *   [col: F0 -> factor c_{s+1} -> (I0 unless s == n-1)]
+ * which is the sequence EV I0 F1 EK F0 I1 EV of single_step_propagation
+ * (wigner_moyal_cuda_1d.py:543-561, rho_vneumann_cuda_1d.py:450-468, and the Bloch steps
+ * wigner_bloch_cuda_1d.py:221-240, rho_bloch_cuda_1d.py:225-246) repeated, with the closing
+ * and opening half-kicks of consecutive steps merged into one column factor.
+ * Column factor c_v = col_line[b][v][j] * col_elem[b][v][i] (v = 0..n), row factor
+ * r_s = row_line[b][s][i] * row_elem[b][s][j]; *_bstride are elements between batch items
+ * (0 shares one table across the batch).  Optional real masks (rows x cols per item):
+ * mask_open_close for columns 0 and n, mask_mid for the merged ones, mask_row for rows.
+ * dscale (complex per item) multiplies the opening pass.  Launches only; no host sync.
+ * const_steps = 1 (time-independent H): the column table holds 3 vectors per item
+ * (0 opening, 1 merged, 2 closing) and the row table 1, reused for every step.
+ * batch > 0 runs only the first `batch` items of the plan (a slot pool's active prefix).
+ */
+typedef struct aw_steps {
+    int nsteps;
+    int const_steps;
+    int batch;
+    const void *col_line, *col_elem, *row_line, *row_elem;
+    long long col_line_bstride, col_elem_bstride, row_line_bstride, row_elem_bstride;
+    const void *mask_open_close, *mask_mid, *mask_row;
+    long long mask_bstride;
+    const double *dscale;
+} aw_steps;
+
+int aw_step_sequence(aw_plan *plan, void *data, const aw_steps *steps, aw_stream stream);
+
+/*
  * Build complex factor vectors in the plan's precision from FP64 inputs:
  *   out[v][k] = scale * s(k) * mag[v][k]^mag_pow * E(c0 + ca * A[v][k] + cb * B[v][k])
  * with E(x) = exp(i x) (mode 0) or exp(x) (mode 1), s(k) = (-1)^k if alt_sign else 1.
