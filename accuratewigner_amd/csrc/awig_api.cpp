@@ -223,7 +223,7 @@ int aw_fft_axis(aw_plan *p, void *data, void *out, int axis, int dir, aw_stream 
     return aw_pass_run(p, data, &ps, stream);
 }
 
-int aw_make_factors(aw_plan *p, void *out, int n, int nvec, int mode, const double *A,
+int aw_make_factors(aw_plan *p, void *out, long long out_stride, int n, int nvec, int mode, const double *A,
                     long long a_stride, double ca, const double *B, long long b_stride, double cb,
                     double c0, const double *mag, long long mag_stride, int mag_pow, double scale,
                     int alt_sign, aw_stream stream) {
@@ -231,8 +231,10 @@ int aw_make_factors(aw_plan *p, void *out, int n, int nvec, int mode, const doub
     if (!out || n < 1 || nvec < 1) return set_error(AW_ERR_INVALID_VALUE, "bad factor arguments");
     if (mode != 0 && mode != 1) return set_error(AW_ERR_INVALID_VALUE, "mode must be 0 or 1");
     if (mag_pow < 0 || mag_pow > 8) return set_error(AW_ERR_INVALID_VALUE, "mag_pow out of range");
+    if (out_stride != 0 && out_stride < n) return set_error(AW_ERR_INVALID_VALUE, "out_stride < n");
     FactorArgs f{};
     f.out = out;
+    f.out_stride = out_stride ? out_stride : n;
     f.n = n;
     f.nvec = nvec;
     f.mode = mode;

@@ -87,7 +87,7 @@ __global__ void k_make_factors(FactorArgs f) {
             re = mag * exp(arg);
             im = 0.0;
         }
-        reinterpret_cast<T2 *>(f.out)[idx] = mk<T2>(re, im);
+        reinterpret_cast<T2 *>(f.out)[vi * f.out_stride + k] = mk<T2>(re, im);
     }
 }
 

@@ -34,7 +34,7 @@ struct PassArgs {
 };
 
 struct FactorArgs {
-    void *out;
+    void *out; long long out_stride;
     int n, nvec, mode;
     const double *A; long long a_stride; double ca;
     const double *B; long long b_stride; double cb;

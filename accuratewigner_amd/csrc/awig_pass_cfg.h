@@ -40,9 +40,27 @@ template <typename R, int N, bool STRIDED> struct PassTune {
     static constexpr int LPB = fit(WANT, TWM);
 };
 
+// Small passes (lines up to 2048 points, at most kSmallLines lines in the whole batch: one
+// 256^2 state, a few 128^2 states, one 1024^2 state) are bound by the latency of one block, not by HBM: fewer
+// elements per thread (E = N/128, 4..16) and 1-2 lines per block spread the lines over
+// many more CUs and shorten each thread's chain.  Twiddles come from LDS.  Measured on
+// MI355X: one 256^2 FP64 state 6.8 -> 4.7 us per pass (the back-to-back launch floor is
+// 2.5 us); no gain once the batch has more than ~2k lines.
+constexpr long long kSmallLines = 2048;
+
+template <typename R, int N, bool STRIDED> struct SmallTune {
+    static constexpr int E = (N / 128 < 4) ? 4 : ((N / 128 > 16) ? 16 : N / 128);
+    static constexpr int T = N / E;
+    static constexpr int LPB = STRIDED ? 2 : (T >= 64 ? 1 : 64 / T);
+};
+
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN>
 using PassK = KCfg<R, N, PassTune<R, N, STRIDED>::E, PassTune<R, N, STRIDED>::LPB, STRIDED,
                    PassTune<R, N, STRIDED>::TWM, DA, DB, GEN>;
+
+template <typename R, int N, bool STRIDED, int DA, int DB, int GEN>
+using PassKS = KCfg<R, N, SmallTune<R, N, STRIDED>::E, SmallTune<R, N, STRIDED>::LPB, STRIDED, 1, DA,
+                    DB, GEN>;
 
 template <typename C>
 int launch_cfg(const PassArgs &a, hipStream_t st) {
@@ -61,6 +79,16 @@ int launch_cfg(const PassArgs &a, hipStream_t st) {
 //       1 step [factor, inverse FFT]; 2 step [forward, factor, inverse]; 3 step [forward, factor]
 template <typename R, int N, bool STRIDED>
 int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
+    if constexpr (N <= 2048) {
+        if (a.total_lines <= kSmallLines) {
+            switch (kind) {
+            case 1: return launch_cfg<PassKS<R, N, STRIDED, 0, 1, 0>>(a, st);
+            case 2: return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0>>(a, st);
+            case 3: return launch_cfg<PassKS<R, N, STRIDED, -1, 0, 0>>(a, st);
+            default: return launch_cfg<PassKS<R, N, STRIDED, 2, 2, 1>>(a, st);
+            }
+        }
+    }
     switch (kind) {
     case 1: return launch_cfg<PassK<R, N, STRIDED, 0, 1, 0>>(a, st);
     case 2: return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0>>(a, st);

@@ -97,19 +97,12 @@ class Plan:
     def factors(self, out, n, nvec, A=None, ca=0.0, B=None, cb=0.0, c0=0.0, mag=None,
                 mag_pow=1, scale=1.0, alt_sign=False, mode=0, a_stride=None, b_stride=None,
                 mag_stride=None, out_offset=0, out_vstride=1):
-        """out[v][k] = scale * (-1)^k? * mag^pow * E(c0 + ca A + cb B), E = exp(i.) (mode 0)
-        or exp(.) (mode 1); A, B, mag are FP64 device pointers (ints) or None."""
+        """out[out_offset + v * out_vstride][k] = scale * (-1)^k? * mag^pow * E(c0 + ca A + cb B),
+        E = exp(i.) (mode 0) or exp(.) (mode 1), in units of length-n vectors; A, B, mag are
+        FP64 device pointers (ints) or None."""
         esz = np.dtype(self.state_dtype).itemsize
-        if out_vstride != 1:
-            # vectors interleaved in the output: one launch per vector
-            for v in range(nvec):
-                self.factors(out, n, 1, A=None if A is None else A + v * (n if a_stride is None else a_stride) * 8,
-                             ca=ca, B=None if B is None else B + v * (n if b_stride is None else b_stride) * 8,
-                             cb=cb, c0=c0, mag=mag, mag_pow=mag_pow, scale=scale, alt_sign=alt_sign,
-                             mode=mode, mag_stride=mag_stride, out_offset=out_offset + v * out_vstride)
-            return
         _lib.call("aw_make_factors", self.handle, ctypes.c_void_p(_p(out) + out_offset * n * esz),
-                  n, nvec, mode,
+                  out_vstride * n, n, nvec, mode,
                   ctypes.c_void_p(A), n if a_stride is None else a_stride, ca,
                   ctypes.c_void_p(B), n if b_stride is None else b_stride, cb,
                   c0, ctypes.c_void_p(mag), 0 if mag_stride is None else mag_stride, mag_pow,

@@ -298,12 +298,11 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
         colL, colE, rowL, rowE = self._chunk_factors(ts, merged=False)
         p, rho = self.plan, self.rho
         for s in range(n):
-            p.run(rho, 0, 0, INV, line_fac=colL.ptr + 2 * s * N * csz, elem_fac=colE.ptr + 2 * s * N * csz,
-                  dscale=self._dscale if self._scale_pending else None)
+            # one step = three passes; its opening / closing vectors sit at 2 s and 2 s + 1
+            p.steps(rho, 1, colL.ptr + 2 * s * N * csz, colE.ptr + 2 * s * N * csz,
+                    rowL.ptr + s * N * csz, rowE.ptr + s * N * csz,
+                    dscale=self._dscale if self._scale_pending else None)
             self._scale_pending = False
-            p.run(rho, 1, FWD, INV, line_fac=rowL.ptr + s * N * csz, elem_fac=rowE.ptr + s * N * csz)
-            p.run(rho, 0, FWD, 0, line_fac=colL.ptr + (2 * s + 1) * N * csz,
-                  elem_fac=colE.ptr + (2 * s + 1) * N * csz)
             observe(s)
             self._scale_pending = True
 

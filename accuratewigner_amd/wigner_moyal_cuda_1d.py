@@ -25,8 +25,9 @@ import numpy as np
 from . import _lib, rtc
 from .device import DeviceArray, current_stream, empty, require_gpu, to_device
 from .engine import FWD, INV, Plan, mem_info, rank1_split, sum_partials
+from . import wm_ops
+from .wm_ops import WMGrid
 
-SQRT2 = np.sqrt(2.0)
 CHUNK_STEPS = 256
 
 
@@ -116,6 +117,7 @@ class WignerMoyalCUDA1D:
             self.X_average_RHS, self.P_average_RHS = [], []
             self.hamiltonian_average = []
         self._observable_kernels = {}
+        self._obs_partials = {}
         self._prepare_masks()
         self.print_memory_info()
 
@@ -123,37 +125,18 @@ class WignerMoyalCUDA1D:
     # grids and compiled physics
     # ------------------------------------------------------------------------------------
     def _make_grids(self):
-        """Grids exactly as wigner_moyal_cuda_1d.py:110-143."""
-        Nx, Np = self.X_gridDIM, self.P_gridDIM
-        Xa, Pa = self.X_amplitude, self.P_amplitude
-        self.dX = 2. * Xa / Nx
-        self.dP = 2. * Pa / Np
-        self.dXdP = self.dX * self.dP
-        self.X = np.linspace(-Xa, Xa - self.dX, Nx)
-        self.P = np.linspace(-Pa, Pa - self.dP, Np)
-        self.PP_prime = SQRT2 * self.P
-        self.dPP_prime = self.PP_prime[1] - self.PP_prime[0]
-        self.XX_prime = np.arange(Np, dtype=np.float64) - Np // 2
-        self.XX_prime *= np.pi / (SQRT2 * Pa)
-        self.dXX_prime = self.XX_prime[1] - self.XX_prime[0]
-        self.XX = SQRT2 * self.X
-        self.dXX = self.XX[1] - self.XX[0]
-        self.PP = np.arange(Nx, dtype=np.float64) - Nx // 2
-        self.PP *= np.pi / (SQRT2 * Xa)
-        self.dPP = self.PP[1] - self.PP[0]
-        self.P = self.P[:, np.newaxis]
-        self.X = self.X[np.newaxis, :]
-        self.XX = self.XX[np.newaxis, :]
-        self.PP = self.PP[np.newaxis, :]
-        self.XX_prime = self.XX_prime[:, np.newaxis]
-        self.PP_prime = self.PP_prime[:, np.newaxis]
-        # coordinates as the reference kernels recompute them from the step constants
-        j = np.arange(Nx, dtype=np.float64)
-        i = np.arange(Np, dtype=np.float64)
-        self._kXX = self.dXX * (j - 0.5 * Nx)
-        self._kXXp = self.dXX_prime * (i - 0.5 * Np)
-        self._kPP = self.dPP * (j - 0.5 * Nx)
-        self._kPPp = self.dPP_prime * (i - 0.5 * Np)
+        """Grids exactly as wigner_moyal_cuda_1d.py:110-143 (1D vectors in WMGrid, exposed here
+        with the reference's broadcast shapes)."""
+        g = self._grid = WMGrid(self.X_gridDIM, self.X_amplitude, self.P_gridDIM, self.P_amplitude)
+        self.dX, self.dP, self.dXdP = g.dX, g.dP, g.dXdP
+        self.dPP_prime, self.dXX_prime, self.dXX, self.dPP = g.dPP_prime, g.dXX_prime, g.dXX, g.dPP
+        self.P = g.P[:, np.newaxis]
+        self.X = g.X[np.newaxis, :]
+        self.XX = g.XX[np.newaxis, :]
+        self.PP = g.PP[np.newaxis, :]
+        self.XX_prime = g.XX_prime[:, np.newaxis]
+        self.PP_prime = g.PP_prime[:, np.newaxis]
+        self._kXX, self._kXXp, self._kPP, self._kPPp = g.kXX, g.kXXp, g.kPP, g.kPPp
 
     def _state_t(self):
         return "double2" if self.state_dtype == np.complex128 else "float2"
@@ -265,7 +248,11 @@ extern "C" __global__ void aw_obs(const %(st)s *W, long long n_item, double t, d
         mod = self._observable_kernel(exprs)
         n = self.X_gridDIM * self.P_gridDIM
         nblk = max(1, min(512, n // 2048))
-        partials = empty((self.batch, nblk, len(exprs)), np.float64)
+        key = (nblk, len(exprs))
+        partials = self._obs_partials.get(key)
+        if partials is None:
+            # reused by every launch of this shape (stream order keeps the uses apart)
+            partials = self._obs_partials[key] = empty((self.batch, nblk, len(exprs)), np.float64)
         mod.launch("aw_obs", (nblk, self.batch), (256,), [W.ptr, ctypes.c_longlong(n), ctypes.c_double(t),
                                                           partials.ptr], current_stream())
         sum_partials(partials, nblk, len(exprs), self.batch, out)
@@ -346,36 +333,22 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         return self
 
     def _shearX(self, data, dir_a, dir_b, **kw):
-        """phase_shearX (:639-654): exp(-i tan(pi/8) PP_j XX'_i) / Nx on row lines."""
-        self.plan.run(data, 1, dir_a, dir_b, chirp=-np.tan(np.pi / 8.),
-                      lstep=self.dXX_prime, lcenter=0.5 * self.P_gridDIM,
-                      estep=self.dPP, ecenter=0.5 * self.X_gridDIM,
-                      scale=1.0 / self.X_gridDIM, **kw)
+        wm_ops.shear_x(self.plan, self._grid, data, dir_a, dir_b, **kw)
 
     def _shearY(self, data, dir_a, dir_b, **kw):
-        """phase_shearY (:656-670): exp(+i sin(pi/4) PP'_i XX_j) / Np on column lines."""
-        self.plan.run(data, 0, dir_a, dir_b, chirp=np.sin(np.pi / 4.),
-                      lstep=self.dXX, lcenter=0.5 * self.X_gridDIM,
-                      estep=self.dPP_prime, ecenter=0.5 * self.P_gridDIM,
-                      scale=1.0 / self.P_gridDIM, **kw)
+        wm_ops.shear_y(self.plan, self._grid, data, dir_a, dir_b, **kw)
 
     def wigner2rho(self):
         """wigner_moyal_cuda_1d.py:422-469 as four passes:
         [S, F0] [I1, SX, F1] [I0, SY, F0] [I1, SX, F1, S]."""
-        self.plan.run(self.wignerfunction, 0, FWD, 0, sign_in=1, out=self.rho)
-        self._shearX(self.rho, INV, FWD)
-        self._shearY(self.rho, INV, FWD)
-        self._shearX(self.rho, INV, FWD, sign_out=1)
+        wm_ops.wigner2rho(self.plan, self._grid, self.wignerfunction, self.rho)
         return self.rho
 
     def rho2wigner_(self):
         """wigner_moyal_cuda_1d.py:471-501 as four passes:
         [S, F1, SX, I1] [F0, SY, I0] [F1, SX, I1] [I0, S]."""
         self._flush_scale()
-        self._shearX(self.rho, FWD, INV, sign_in=1, out=self.wignerfunction)
-        self._shearY(self.wignerfunction, FWD, INV)
-        self._shearX(self.wignerfunction, FWD, INV)
-        self.plan.run(self.wignerfunction, 0, INV, 0, sign_out=1)
+        wm_ops.rho2wigner(self.plan, self._grid, self.rho, self.wignerfunction)
         return self.wignerfunction
 
     rho2wigner = rho2wigner_
@@ -438,8 +411,11 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             return m[1], m[2], None          # (v over lines-i, u over cols-j)
         return None, None, (m[1] if power == 1 else m[2])
 
-    def _build_chunk_factors(self, ts, first_dscale):
-        """Phase tables (FP64, hipRTC) and the per-pass complex factor vectors of one chunk."""
+    def _build_chunk_factors(self, ts, merged=True):
+        """Phase tables (FP64, hipRTC) and the per-pass complex factor vectors of one chunk.
+        merged: the closing half-kick of a step and the opening one of the next share one
+        column pass (n + 1 column vectors); otherwise each step has its own opening and
+        closing vectors (2 n column vectors, open at 2 s, close at 2 s + 1)."""
         n = len(ts)
         Nx, Np, dt = self.X_gridDIM, self.P_gridDIM, self.dt
         p = self.plan
@@ -456,10 +432,23 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         mpp, mp, _ = self._mask_args(1, 1)
         mxp_p = mxp.ptr if mxp is not None else None
         mx_p = mx.ptr if mx is not None else None
-        colL = p.new_factors(Nx, n + 1)
-        colE = p.new_factors(Np, n + 1)
         rowL = p.new_factors(Np, n)
         rowE = p.new_factors(Nx, n)
+        # row passes: exp(-i dt [K(PP_j) - K(PP'_i)]) absp / Nx
+        p.factors(rowL, Np, n, A=Ki.ptr, ca=dt, mag=mpp.ptr if mpp is not None else None,
+                  scale=1.0 / Nx)
+        p.factors(rowE, Nx, n, A=Kj.ptr, ca=-dt, mag=mp.ptr if mp is not None else None)
+        if not merged:
+            colL = p.new_factors(Nx, 2 * n)
+            colE = p.new_factors(Np, 2 * n)
+            for v, scale in ((0, 1.0), (1, 1.0 / Np)):
+                p.factors(colL, Nx, n, A=Vj.ptr, ca=-0.5 * dt, mag=mx_p, alt_sign=True,
+                          out_offset=v, out_vstride=2)
+                p.factors(colE, Np, n, A=Vi.ptr, ca=0.5 * dt, mag=mxp_p, alt_sign=True, scale=scale,
+                          out_offset=v, out_vstride=2)
+            return colL, colE, rowL, rowE
+        colL = p.new_factors(Nx, n + 1)
+        colE = p.new_factors(Np, n + 1)
         # column passes: v = 0 start EV(t_0) (with S), 1..n-1 merged EV EV', n end EV (with S)
         p.factors(colL, Nx, 1, A=Vj.ptr, ca=-0.5 * dt, mag=mx_p, alt_sign=True)
         p.factors(colE, Np, 1, A=Vi.ptr, ca=0.5 * dt, mag=mxp_p, alt_sign=True)
@@ -472,17 +461,13 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
                   out_offset=n)
         p.factors(colE, Np, 1, A=Vi.ptr + (n - 1) * Np * 8, ca=0.5 * dt, mag=mxp_p, alt_sign=True,
                   scale=1.0 / Np, out_offset=n)
-        # row passes: exp(-i dt [K(PP_j) - K(PP'_i)]) absp / Nx
-        p.factors(rowL, Np, n, A=Ki.ptr, ca=dt, mag=mpp.ptr if mpp is not None else None,
-                  scale=1.0 / Nx)
-        p.factors(rowE, Nx, n, A=Kj.ptr, ca=-dt, mag=mp.ptr if mp is not None else None)
         return colL, colE, rowL, rowE
 
     def _run_chunk(self, ts, dscale=None):
         """Steps with evaluation times ts (t after the increment) as 2 n + 1 fused passes."""
         n = len(ts)
         Nx, Np = self.X_gridDIM, self.P_gridDIM
-        colL, colE, rowL, rowE = self._build_chunk_factors(ts, dscale)
+        colL, colE, rowL, rowE = self._build_chunk_factors(ts)
         csz = np.dtype(self.state_dtype).itemsize
         _, _, tab1 = self._mask_args(0, 1)
         _, _, tab2 = self._mask_args(0, 2)
@@ -527,13 +512,25 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         nobs = len(exprs)
         ring = empty((steps, self.batch, nobs), np.float64)
         row_bytes = self.batch * nobs * 8
-        for s in range(steps):
-            self._run_chunk(ts[s:s + 1], dscale=self._dscale if self._scale_pending else None)
-            self._scale_pending = False
-            self.rho2wigner_()
-            self._obs_launch(exprs, self.wignerfunction, float(ts[s]), ring.ptr + s * row_bytes)
-            self.plan.scale_from_stats(ring.ptr + s * row_bytes, nobs, 0, 0, self.dXdP, self._dscale)
-            self._scale_pending = True
+        _, _, tab1 = self._mask_args(0, 1)
+        _, _, tabp = self._mask_args(1, 1)
+        p, rho = self.plan, self.rho
+        csz = np.dtype(self.state_dtype).itemsize
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        for c0 in range(0, steps, CHUNK_STEPS):
+            tsc = ts[c0:c0 + CHUNK_STEPS]
+            colL, colE, rowL, rowE = self._build_chunk_factors(tsc, merged=False)
+            for k in range(len(tsc)):
+                s = c0 + k
+                # one step = three passes; its opening / closing vectors sit at 2 k and 2 k + 1
+                p.steps(rho, 1, colL.ptr + 2 * k * Nx * csz, colE.ptr + 2 * k * Np * csz,
+                        rowL.ptr + k * Np * csz, rowE.ptr + k * Nx * csz, mask_open_close=tab1,
+                        mask_row=tabp, dscale=self._dscale if self._scale_pending else None)
+                self._scale_pending = False
+                self.rho2wigner_()
+                self._obs_launch(exprs, self.wignerfunction, float(ts[s]), ring.ptr + s * row_bytes)
+                self.plan.scale_from_stats(ring.ptr + s * row_bytes, nobs, 0, 0, self.dXdP, self._dscale)
+                self._scale_pending = True
         self.t = ts[-1]
         # final normalisation of rho and W by the last step's norm
         self.plan.scale(self.wignerfunction, self._dscale)

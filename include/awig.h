@@ -129,14 +129,15 @@ int aw_step_sequence(aw_plan *plan, void *data, const aw_steps *steps, aw_stream
 
 /*
  * Build complex factor vectors in the plan's precision from FP64 inputs:
- *   out[v][k] = scale * s(k) * mag[v][k]^mag_pow * E(c0 + ca * A[v][k] + cb * B[v][k])
+ *   out[v * out_stride + k] = scale * s(k) * mag[v][k]^mag_pow * E(c0 + ca * A[v][k] + cb * B[v][k])
  * with E(x) = exp(i x) (mode 0) or exp(x) (mode 1), s(k) = (-1)^k if alt_sign else 1.
- * nvec vectors of length n; A, B, mag may be NULL (zero / zero / one) and have strides
+ * nvec vectors of length n, out_stride elements apart (0 means n, i.e. packed; a larger
+ * stride interleaves several tables); A, B, mag may be NULL (zero / zero / one) and have strides
  * a_stride, b_stride, mag_stride (0 broadcasts one vector to all nvec).  Phase arguments
  * are summed and exponentiated in FP64 (the reference evaluates every phase in double,
  * e.g. wigner_moyal_cuda_1d.py:751, 772); only the result is rounded to the plan precision.
  */
-int aw_make_factors(aw_plan *plan, void *out, int n, int nvec, int mode,
+int aw_make_factors(aw_plan *plan, void *out, long long out_stride, int n, int nvec, int mode,
                     const double *A, long long a_stride, double ca,
                     const double *B, long long b_stride, double cb,
                     double c0, const double *mag, long long mag_stride, int mag_pow,

@@ -27,14 +27,17 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("big", [False, True])
 @pytest.mark.parametrize("n", [64, 128, 256, 512, 1024, 2048])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_fft_axis_matches_numpy(n, prec):
+def test_fft_axis_matches_numpy(n, prec, big):
+    """Both kernel configurations: small batches (at most 2048 lines per pass, latency
+    configuration) and batches above that (throughput configuration)."""
     import torch
     from accuratewigner_amd.engine import Plan, FWD, INV
     from accuratewigner_amd.device import to_device
     rows, cols = (n, 128) if n != 128 else (128, 256)
-    batch = 2
+    batch = 2048 // min(rows, cols) + 1 if big else 2
     rng = np.random.default_rng(n)
     a = rng.normal(size=(batch, rows, cols)) + 1j * rng.normal(size=(batch, rows, cols))
     dt = np.complex128 if prec == "fp64" else np.complex64
@@ -132,6 +135,25 @@ def test_wm_mixed_precision_error_budget():
     assert W.dtype == np.complex64
     err = rel(W.astype(np.complex128), o.propagate(200))
     assert err < 2e-5, err
+
+
+def test_wm_large_batch_fp64_matches_oracle():
+    """A batch above the small-pass threshold (10 x 256 lines) runs the throughput kernel
+    configuration; every state matches the oracle."""
+    from accuratewigner_amd import WignerMoyalCUDA1D
+    kw = dict(X_gridDIM=256, X_amplitude=8., P_gridDIM=256, P_amplitude=8., dt=0.01)
+    g = WignerMoyalCUDA1D(t=0., K="0.5 * P * P", V="0.5 * X * X + 0.1 * X * X * X * X",
+                          precision="fp64", batch=10, **kw)
+    x0 = np.linspace(-1.5, 1.5, 10)
+    X, P = g.X, g.P
+    W0 = np.exp(-(X[None] - x0[:, None, None]) ** 2 - (P[None] - 0.3) ** 2) / np.pi
+    g.set_wignerfunction(W0)
+    Wg = g.propagate(30).get()
+    for k in (0, 4, 9):
+        o = WignerMoyalOracle(V=lambda X, t: 0.5 * X * X + 0.1 * X ** 4, K=lambda P, t: 0.5 * P * P, **kw)
+        o.set_wignerfunction(W0[k])
+        o.propagate(30)
+        assert rel(Wg[k], o.wignerfunction) < 1e-10, k
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
