@@ -254,6 +254,14 @@ int aw_reduce_stats(aw_plan *p, const void *data, double *out, aw_stream stream)
                         p->partials, p->red_blocks, out, (hipStream_t)stream);
 }
 
+int aw_pair_dot(aw_plan *p, const void *a, const void *b, const double *u, const double *v, double *out,
+                aw_stream stream) {
+    if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
+    if (!a || !b || !u || !v || !out) return set_error(AW_ERR_INVALID_VALUE, "NULL pair-dot argument");
+    return launch_pair_dot(p->prec, a, b, (long long)p->rows * p->cols, p->cols, u, v, p->batch, p->partials,
+                           p->red_blocks, out, (hipStream_t)stream);
+}
+
 int aw_sum_partials(const double *partials, int nblk, int ncomp, int batch, double *out,
                     aw_stream stream) {
     if (!partials || !out || nblk < 1 || ncomp < 1 || batch < 1)

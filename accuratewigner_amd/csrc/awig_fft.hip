@@ -112,6 +112,8 @@ int launch_make_factors(int prec, const FactorArgs &f, hipStream_t st) {
 constexpr int RED_THREADS = 256;
 constexpr int NSTAT = 8;
 
+__global__ void k_sum_partials(const double *partials, int nblk, int ncomp, double *out);
+
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
@@ -159,6 +161,46 @@ __global__ __launch_bounds__(RED_THREADS) void k_stats(const void *data, long lo
         acc[6] += q.y;
     }
     block_sum_store<NSTAT>(acc, partials + ((long long)b * nblk + blockIdx.x) * NSTAT);
+}
+
+// weighted inner products of two states: per item, sum conj(a) b u[j] and sum conj(a) b v[i]
+// (4 doubles: Re, Im of each); the adjoint's gradient of a dipole coupling
+template <typename R>
+__global__ __launch_bounds__(RED_THREADS) void k_pair_dot(const void *a, const void *bb, long long n_item,
+                                                            int cols, const double *u, const double *v,
+                                                            double *partials, int nblk) {
+    using T2 = typename Cx<R>::T;
+    const int b = blockIdx.y;
+    const T2 *za = reinterpret_cast<const T2 *>(a) + (long long)b * n_item;
+    const T2 *zb = reinterpret_cast<const T2 *>(bb) + (long long)b * n_item;
+    double acc[4] = {0, 0, 0, 0};
+    const long long stride = (long long)nblk * RED_THREADS;
+    for (long long i = (long long)blockIdx.x * RED_THREADS + threadIdx.x; i < n_item; i += stride) {
+        const T2 p = za[i], q = zb[i];
+        const double re = double(p.x) * q.x + double(p.y) * q.y;   // conj(p) q
+        const double im = double(p.x) * q.y - double(p.y) * q.x;
+        const int ii = int(i), r = ii / cols;   // an item holds at most 4096^2 elements
+        const double uj = u[ii - r * cols], vi = v[r];
+        acc[0] += re * uj;
+        acc[1] += im * uj;
+        acc[2] += re * vi;
+        acc[3] += im * vi;
+    }
+    block_sum_store<4>(acc, partials + ((long long)b * nblk + blockIdx.x) * 4);
+}
+
+int launch_pair_dot(int prec, const void *a, const void *b, long long n_item, int cols, const double *u,
+                    const double *v, int batch, double *partials, int nblk, double *out, hipStream_t st) {
+    if (prec == AW_PREC_F64)
+        hipLaunchKernelGGL(k_pair_dot<double>, dim3(nblk, batch), dim3(RED_THREADS), 0, st, a, b, n_item, cols,
+                           u, v, partials, nblk);
+    else
+        hipLaunchKernelGGL(k_pair_dot<float>, dim3(nblk, batch), dim3(RED_THREADS), 0, st, a, b, n_item, cols,
+                           u, v, partials, nblk);
+    hipLaunchKernelGGL(k_sum_partials, dim3(batch), dim3(64), 0, st, partials, nblk, 4, out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_pair_dot: %s", hipGetErrorString(e));
+    return AW_OK;
 }
 
 __global__ void k_sum_partials(const double *partials, int nblk, int ncomp, double *out) {

@@ -119,6 +119,13 @@ class Plan:
                   ctypes.c_void_p(current_stream()))
         return out
 
+    def pair_dot(self, a, b, u, v, out):
+        """out[batch, 4] (device FP64) = Re, Im sum conj(a) b u_j and Re, Im sum conj(a) b v_i."""
+        _lib.call("aw_pair_dot", self.handle, ctypes.c_void_p(_p(a)), ctypes.c_void_p(_p(b)),
+                  ctypes.c_void_p(_p(u)), ctypes.c_void_p(_p(v)), ctypes.c_void_p(_p(out)),
+                  ctypes.c_void_p(current_stream()))
+        return out
+
     def scale_from_stats(self, stats_ptr, stride, comp, mode, factor, dscale, batch=None):
         _lib.call("aw_scale_from_stats", ctypes.c_void_p(stats_ptr), stride, comp, mode, factor,
                   ctypes.c_void_p(_p(dscale)), self.batch if batch is None else batch,

@@ -418,7 +418,6 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         closing vectors (2 n column vectors, open at 2 s, close at 2 s + 1)."""
         n = len(ts)
         Nx, Np, dt = self.X_gridDIM, self.P_gridDIM, self.dt
-        p = self.plan
         ts_dev = to_device(ts)
         Vj = empty((n, Nx), np.float64)
         Vi = empty((n, Np), np.float64)
@@ -428,6 +427,14 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         self._eval_1d(0, self._dev_kXXp, Np, ts_dev, n, 0.5 * dt, Vi)
         self._eval_1d(1, self._dev_kPP, Nx, ts_dev, n, 0.5 * dt, Kj)
         self._eval_1d(1, self._dev_kPPp, Np, ts_dev, n, 0.5 * dt, Ki)
+        return self._factors_from_tables(n, Vj.ptr, Vi.ptr, Kj.ptr, Ki.ptr, dt, merged)
+
+    def _factors_from_tables(self, n, Vj, Vi, Kj, Ki, dt, merged=True, k_stride=None):
+        """Factor vectors of n steps from FP64 phase tables (device pointers) V(XX_j),
+        V(XX'_i) [n, Nx] / [n, Np] and K(PP_j), K(PP'_i) (k_stride 0: one row for all
+        steps).  A negative dt gives the conjugated factors: the adjoint step."""
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        p = self.plan
         mxp, mx, _ = self._mask_args(0, 1)
         mpp, mp, _ = self._mask_args(1, 1)
         mxp_p = mxp.ptr if mxp is not None else None
@@ -435,31 +442,31 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         rowL = p.new_factors(Np, n)
         rowE = p.new_factors(Nx, n)
         # row passes: exp(-i dt [K(PP_j) - K(PP'_i)]) absp / Nx
-        p.factors(rowL, Np, n, A=Ki.ptr, ca=dt, mag=mpp.ptr if mpp is not None else None,
-                  scale=1.0 / Nx)
-        p.factors(rowE, Nx, n, A=Kj.ptr, ca=-dt, mag=mp.ptr if mp is not None else None)
+        p.factors(rowL, Np, n, A=Ki, ca=dt, mag=mpp.ptr if mpp is not None else None,
+                  scale=1.0 / Nx, a_stride=k_stride)
+        p.factors(rowE, Nx, n, A=Kj, ca=-dt, mag=mp.ptr if mp is not None else None, a_stride=k_stride)
         if not merged:
             colL = p.new_factors(Nx, 2 * n)
             colE = p.new_factors(Np, 2 * n)
             for v, scale in ((0, 1.0), (1, 1.0 / Np)):
-                p.factors(colL, Nx, n, A=Vj.ptr, ca=-0.5 * dt, mag=mx_p, alt_sign=True,
+                p.factors(colL, Nx, n, A=Vj, ca=-0.5 * dt, mag=mx_p, alt_sign=True,
                           out_offset=v, out_vstride=2)
-                p.factors(colE, Np, n, A=Vi.ptr, ca=0.5 * dt, mag=mxp_p, alt_sign=True, scale=scale,
+                p.factors(colE, Np, n, A=Vi, ca=0.5 * dt, mag=mxp_p, alt_sign=True, scale=scale,
                           out_offset=v, out_vstride=2)
             return colL, colE, rowL, rowE
         colL = p.new_factors(Nx, n + 1)
         colE = p.new_factors(Np, n + 1)
         # column passes: v = 0 start EV(t_0) (with S), 1..n-1 merged EV EV', n end EV (with S)
-        p.factors(colL, Nx, 1, A=Vj.ptr, ca=-0.5 * dt, mag=mx_p, alt_sign=True)
-        p.factors(colE, Np, 1, A=Vi.ptr, ca=0.5 * dt, mag=mxp_p, alt_sign=True)
+        p.factors(colL, Nx, 1, A=Vj, ca=-0.5 * dt, mag=mx_p, alt_sign=True)
+        p.factors(colE, Np, 1, A=Vi, ca=0.5 * dt, mag=mxp_p, alt_sign=True)
         if n > 1:
-            p.factors(colL, Nx, n - 1, A=Vj.ptr, ca=-0.5 * dt, B=Vj.ptr + Nx * 8, cb=-0.5 * dt,
+            p.factors(colL, Nx, n - 1, A=Vj, ca=-0.5 * dt, B=Vj + Nx * 8, cb=-0.5 * dt,
                       mag=mx_p, mag_pow=2, out_offset=1)
-            p.factors(colE, Np, n - 1, A=Vi.ptr, ca=0.5 * dt, B=Vi.ptr + Np * 8, cb=0.5 * dt,
+            p.factors(colE, Np, n - 1, A=Vi, ca=0.5 * dt, B=Vi + Np * 8, cb=0.5 * dt,
                       mag=mxp_p, mag_pow=2, scale=1.0 / Np, out_offset=1)
-        p.factors(colL, Nx, 1, A=Vj.ptr + (n - 1) * Nx * 8, ca=-0.5 * dt, mag=mx_p, alt_sign=True,
+        p.factors(colL, Nx, 1, A=Vj + (n - 1) * Nx * 8, ca=-0.5 * dt, mag=mx_p, alt_sign=True,
                   out_offset=n)
-        p.factors(colE, Np, 1, A=Vi.ptr + (n - 1) * Np * 8, ca=0.5 * dt, mag=mxp_p, alt_sign=True,
+        p.factors(colE, Np, 1, A=Vi + (n - 1) * Np * 8, ca=0.5 * dt, mag=mxp_p, alt_sign=True,
                   scale=1.0 / Np, out_offset=n)
         return colL, colE, rowL, rowE
 

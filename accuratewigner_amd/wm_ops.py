@@ -41,15 +41,19 @@ class WMGrid:
         self.kP = self.dP * (i - 0.5 * Np)
 
 
-def shear_x(plan, g, data, dir_a, dir_b, **kw):
-    """phase_shearX (wigner_moyal_cuda_1d.py:639-654): exp(-i tan(pi/8) PP_j XX'_i) / Nx on rows."""
-    plan.run(data, 1, dir_a, dir_b, chirp=-np.tan(np.pi / 8.), lstep=g.dXX_prime, lcenter=0.5 * g.Np,
+def shear_x(plan, g, data, dir_a, dir_b, conj=False, **kw):
+    """phase_shearX (wigner_moyal_cuda_1d.py:639-654): exp(-i tan(pi/8) PP_j XX'_i) / Nx on rows
+    (conj: the complex conjugate factor)."""
+    c = -np.tan(np.pi / 8.)
+    plan.run(data, 1, dir_a, dir_b, chirp=-c if conj else c, lstep=g.dXX_prime, lcenter=0.5 * g.Np,
              estep=g.dPP, ecenter=0.5 * g.Nx, scale=1.0 / g.Nx, **kw)
 
 
-def shear_y(plan, g, data, dir_a, dir_b, **kw):
-    """phase_shearY (wigner_moyal_cuda_1d.py:656-670): exp(+i sin(pi/4) PP'_i XX_j) / Np on columns."""
-    plan.run(data, 0, dir_a, dir_b, chirp=np.sin(np.pi / 4.), lstep=g.dXX, lcenter=0.5 * g.Nx,
+def shear_y(plan, g, data, dir_a, dir_b, conj=False, **kw):
+    """phase_shearY (wigner_moyal_cuda_1d.py:656-670): exp(+i sin(pi/4) PP'_i XX_j) / Np on columns
+    (conj: the complex conjugate factor)."""
+    c = np.sin(np.pi / 4.)
+    plan.run(data, 0, dir_a, dir_b, chirp=-c if conj else c, lstep=g.dXX, lcenter=0.5 * g.Nx,
              estep=g.dPP_prime, ecenter=0.5 * g.Np, scale=1.0 / g.Np, **kw)
 
 
@@ -68,3 +72,13 @@ def rho2wigner(plan, g, rho, W):
     shear_y(plan, g, W, FWD, INV)
     shear_x(plan, g, W, FWD, INV)
     plan.run(W, 0, INV, 0, sign_out=1)
+
+
+def rho2wigner_adjoint(plan, g, data):
+    """Conjugate transpose of rho2wigner, in place: with F^H = I (unnormalised DFTs) and
+    D^H = conj(D) for the sign and shear factors, the reversed sequence is
+    [S, F0] [F1, SX*, I1] [F0, SY*, I0] [F1, SX*, I1, S].  Re<a, L b> = Re<L^H a, b>."""
+    plan.run(data, 0, FWD, 0, sign_in=1)
+    shear_x(plan, g, data, FWD, INV, conj=True)
+    shear_y(plan, g, data, FWD, INV, conj=True)
+    shear_x(plan, g, data, FWD, INV, conj=True, sign_out=1)

@@ -154,6 +154,16 @@ int aw_make_factors(aw_plan *plan, void *out, long long out_stride, int n, int n
 int aw_reduce_stats(aw_plan *plan, const void *data, double *out, aw_stream stream);
 
 /*
+ * Weighted inner products of two states of the plan's shape, per batch item (FP64):
+ *   out[b][0..3] = { Re, Im of sum_ij conj(a_ij) b_ij u_j,  Re, Im of sum_ij conj(a_ij) b_ij v_i }
+ * u has cols entries, v has rows entries (device FP64).  No reference counterpart: the
+ * gradient of the adjoint with respect to a dipole field (the derivative of the expV
+ * factor, wigner_moyal_cuda_1d.py:763-778, is -i dt/2 (XX_j - XX'_i) times that factor).
+ */
+int aw_pair_dot(aw_plan *plan, const void *a, const void *b, const double *u, const double *v,
+                double *out, aw_stream stream);
+
+/*
  * out[b][c] = sum_i partials[b][i][c] for i < nblk, c < ncomp, summed in fixed order (the
  * second stage of every block-partial reduction, including the hipRTC observable kernels).
  */
