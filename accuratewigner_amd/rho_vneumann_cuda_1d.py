@@ -78,8 +78,9 @@ class RhoVNeumannCUDA1D:
         self.X_wigner = self.X_wigner[np.newaxis, :]
         self.P_wigner = self.P_wigner[:, np.newaxis]
         j = np.arange(N, dtype=np.float64)
-        self._kX = self.dX * (j - 0.5 * N)
-        self._kP = self.dP * (j - 0.5 * N)
+        # kernel coordinates from the #define'd steps (X = dX * (j - 0.5 * X_gridDIM))
+        self._kX = rtc.kconst(self.dX) * (j - 0.5 * N)
+        self._kP = rtc.kconst(self.dP) * (j - 0.5 * N)
 
         kwargs.update(dX=self.dX, dP=self.dP, XX_gridDIM=self.XX_gridDIM, dXX=self.dX,
                       dPP=self.dPP, P_max=self.P.max(), X_max=self.X.max())
@@ -498,8 +499,9 @@ extern "C" __global__ void aw_copy_scalar(const double *src, double *dst) { *dst
         W.tensor[pad:pad + N, pad:pad + N] = self.rho.tensor
         wp = self._wplan
         a, b = -np.tan(np.pi / 8.), np.sin(np.pi / 4.)
-        shx = dict(chirp=a, lstep=self.dX, lcenter=0.5 * M, estep=self.dPP, ecenter=0.5 * M, scale=1.0 / M)
-        shy = dict(chirp=b, lstep=self.dX, lcenter=0.5 * M, estep=self.dPP, ecenter=0.5 * M, scale=1.0 / M)
+        kdx, kdpp = rtc.kconst(self.dX), rtc.kconst(self.dPP)
+        shx = dict(chirp=a, lstep=kdx, lcenter=0.5 * M, estep=kdpp, ecenter=0.5 * M, scale=1.0 / M)
+        shy = dict(chirp=b, lstep=kdx, lcenter=0.5 * M, estep=kdpp, ecenter=0.5 * M, scale=1.0 / M)
         wp.run(W, 1, FWD, INV, sign_in=1, **shx)
         wp.run(W, 0, FWD, INV, **shy)
         wp.run(W, 1, FWD, INV, **shx)

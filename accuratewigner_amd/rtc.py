@@ -78,6 +78,14 @@ __device__ inline double aw_wave_sum(double x) {
 """
 
 
+def kconst(x):
+    """The value a kernel sees for a float passed as ``#define name %.15e``
+    (wigner_moyal_cuda_1d.py:174-176): 16 significant digits, so a computed step such as
+    dXX can differ from the host double in its last bit.  Kernel coordinates are built from
+    these values, as the reference kernels build them from their #defines."""
+    return float("%.15e" % float(x))
+
+
 def cuda_consts(kwargs, consts=None, functions=None):
     """``#define`` block of int / float keywords (wigner_moyal_cuda_1d.py:169-182), then
     the user's ``consts`` and ``functions`` strings verbatim (:179-182, :233-237; ``{{ }}``

@@ -69,8 +69,9 @@ class SchrodingerWignerCUDA1D:
         self.dPP = self.P_wigner[1] - self.P_wigner[0]
         self.wigner_dXdP = self.dX * self.dP
         j = np.arange(N)
-        self._kX = self.dX * (j - 0.5 * N)
-        self._kP = self.dP * (((j + N // 2) % N) - 0.5 * N)
+        # kernel coordinates from the #define'd steps (:803, :821)
+        self._kX = rtc.kconst(self.dX) * (j - 0.5 * N)
+        self._kP = rtc.kconst(self.dP) * (((j + N // 2) % N) - 0.5 * N)
 
         kwargs.update(dX=self.dX, dP=self.dP, P_max=self.P.max())
         self.cuda_consts = rtc.cuda_consts(kwargs, getattr(self, "consts", None), None)
@@ -471,9 +472,10 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
         self._mod.launch("aw_outer", (blocks,), (threads,), [self.wavefunction.ptr, W.ptr, ctypes.c_int(N)],
                          current_stream())
         a, b = -np.tan(np.pi / 8.), np.sin(np.pi / 4.)
-        shx = dict(chirp=a, lstep=self.dX, lcenter=0.5 * N, estep=self.dP, ecenter=0.5 * N, efft=1,
+        kdx, kdp = rtc.kconst(self.dX), rtc.kconst(self.dP)
+        shx = dict(chirp=a, lstep=kdx, lcenter=0.5 * N, estep=kdp, ecenter=0.5 * N, efft=1,
                    scale=1.0 / N)
-        shy = dict(chirp=b, lstep=self.dX, lcenter=0.5 * N, estep=self.dP, ecenter=0.5 * N, efft=1,
+        shy = dict(chirp=b, lstep=kdx, lcenter=0.5 * N, estep=kdp, ecenter=0.5 * N, efft=1,
                    scale=1.0 / N)
         wp.run(W, 1, FWD, INV, **shx)
         wp.run(W, 0, FWD, INV, **shy)

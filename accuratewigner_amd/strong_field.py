@@ -1,0 +1,170 @@
+"""Strong-field ionisation driver (strong_field_physics_cuda_1d.py:24-374) on the MI355X path.
+
+The reference script builds a WignerBlochCUDA1D system for a soft-core atom in a laser
+field, finds its ground state (cached in ``ground_state_rho.pickle``), propagates it in
+frames of 100 steps with Ehrenfest lists, stores every frame's Wigner function, time and
+density-matrix diagonal in ``strong_field_physics.hdf5`` and finally plots the high-harmonic
+spectrum |FFT(blackman * <P>)|^2.
+
+Here the same flow runs without plotting.  h5py is not available, so results go to a
+``.npz`` archive that keeps the HDF5 layout as key paths (``settings/<name>``,
+``settings/X_wigner``, ``settings/P_wigner``, ``frames/<n>/{wigner,t,prob}``,
+``ehrenfest/{X_average,P_average,X_average_RHS,P_average_RHS,hamiltonian_average}``,
+:113-125, :247-257, :368-373), written incrementally so the frames never sit in host
+memory together.  The ground-state cache is ``ground_state_rho.npy`` (no pickle).
+"""
+import os
+import zipfile
+
+import numpy as np
+
+
+def sys_params(X_gridDIM=2 * 1024, P_gridDIM=512, X_amplitude=150., P_amplitude=20., dt=0.01):
+    """The reference's parameter set (strong_field_physics_cuda_1d.py:24-85); grid sizes may
+    be reduced for tests."""
+    return dict(
+        t=0.,
+        dt=dt,
+        X_gridDIM=X_gridDIM,
+        X_amplitude=X_amplitude,
+        P_gridDIM=P_gridDIM,
+        P_amplitude=P_amplitude,
+        _gamma=0.01,
+        omega=0.05698,
+        F=0.04,
+        Ip=0.59,
+        functions="""
+    // The laser field (on for 8 periods of the carrier)
+    __device__ double E(double t)
+    {{
+        return -F * sin(omega * t) * pow(sin(omega * t / 16.), 2);
+    }}
+    """,
+        abs_boundary_x="pow("
+                       "    abs(sin(0.5 * M_PI * (XX + XX_max) / XX_max)"
+                       "    * sin(0.5 * M_PI * (XX_prime + XX_prime_max) / XX_prime_max))"
+                       ", dt * 0.05)",
+        E=lambda self, t: -self.F * np.sin(self.omega * t) * np.sin(self.omega * t / 16.) ** 2,
+        K="0.5 * P * P",
+        diff_K="P",
+        V="-1. / sqrt(X * X + 1.37) + X * E(t)",
+        diff_V="X / pow(X * X + 1.37, 1.5) + E(t)",
+    )
+
+
+class NpzWriter:
+    """Incremental ``.npz`` writer: every ``add`` appends one ``<key>.npy`` member."""
+
+    def __init__(self, path):
+        self.path = path
+        self._zf = zipfile.ZipFile(path, "w", compression=zipfile.ZIP_STORED, allowZip64=True)
+
+    def add(self, key, value):
+        arr = np.asanyarray(value)
+        if arr.dtype == object:
+            raise TypeError("%s: not an array value" % key)
+        with self._zf.open(key + ".npy", "w", force_zip64=True) as f:
+            # (np.ascontiguousarray would turn a 0-d scalar into shape (1,))
+            np.lib.format.write_array(f, arr if arr.flags.c_contiguous else np.array(arr, order="C"),
+                                      allow_pickle=False)
+
+    def close(self):
+        self._zf.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def hhg_spectrum(P_average, dt, omega):
+    """High-harmonic spectrum as in strong_field_physics_cuda_1d.py:334-358:
+    |fftshift(FFT(blackman(N) <P>))|^2 normalised to its maximum, against frequency / omega."""
+    J = np.asarray(P_average, dtype=np.float64)
+    N = J.shape[0]
+    spectrum = np.abs(np.fft.fftshift(np.fft.fft(np.blackman(N) * J))) ** 2
+    omegas = np.fft.fftshift(np.fft.fftfreq(N, dt / (2 * np.pi))) / omega
+    return omegas, spectrum / spectrum.max()
+
+
+class StrongFieldRun:
+    """VisualizeDynamicsPhaseSpace without the figure (strong_field_physics_cuda_1d.py:88-269)."""
+
+    def __init__(self, params=None, writer=None, ground_state_file="ground_state_rho.npy",
+                 num_iteration=100, abs_tol_purity=1e-10, precision="fp64", ground_state_max_steps=None):
+        from .wigner_bloch_cuda_1d import WignerBlochCUDA1D
+        self.sys_params = dict(params if params is not None else sys_params())
+        self.writer = writer
+        self.quant_sys = WignerBlochCUDA1D(precision=precision, **self.sys_params)
+        qs = self.quant_sys
+        # final time: 8 periods of the carrier (:188-197)
+        self.T_final = 8 * 2 * np.pi / qs.omega
+        self.num_iteration = int(num_iteration)
+        self.num_frames = int(np.ceil(self.T_final / qs.dt / self.num_iteration))
+        self.current_frame_num = 0
+        self.times = [qs.t]
+        # ground state, cached (:202-214)
+        loaded = False
+        if ground_state_file and os.path.exists(ground_state_file):
+            rho = np.load(ground_state_file, allow_pickle=False)
+            if rho.shape == qs.rho.shape:
+                qs.rho[:] = rho
+                loaded = True
+        if not loaded:
+            qs.get_ground_state(abs_tol_purity=abs_tol_purity, max_steps=ground_state_max_steps)
+            if ground_state_file:
+                np.save(ground_state_file, qs.rho.get(), allow_pickle=False)
+        self.ground_state_loaded = loaded
+        self.wigner_initial = qs.wignerfunction.get().real
+        if writer is not None:
+            for key, val in self.sys_params.items():
+                try:
+                    writer.add("settings/" + key, val)
+                except (TypeError, ValueError):
+                    pass      # functions and the like (h5py raises TypeError, :115-120)
+            writer.add("settings/X_wigner", np.broadcast_to(qs.X, qs.X.shape))
+            writer.add("settings/P_wigner", np.broadcast_to(qs.P, qs.P.shape))
+
+    def frame(self):
+        """One frame (:236-269): propagate num_iteration steps, store W, t and diag(rho)."""
+        qs = self.quant_sys
+        qs.propagate(self.num_iteration)
+        wigner = qs.wignerfunction.get().real
+        prob = qs.rho.tensor.diagonal().cpu().numpy()
+        if self.writer is not None:
+            key = "frames/%d/" % self.current_frame_num
+            self.writer.add(key + "wigner", wigner)
+            self.writer.add(key + "t", np.float64(qs.t))
+            self.writer.add(key + "prob", prob)
+        self.current_frame_num += 1
+        self.times.append(qs.t)
+        return wigner, prob
+
+    def run(self, max_frames=881):
+        for _ in range(min(int(max_frames), self.num_frames)):
+            self.frame()
+        return self
+
+    def save_ehrenfest(self):
+        """:368-373."""
+        qs = self.quant_sys
+        if self.writer is not None:
+            for name in ("X_average", "P_average", "X_average_RHS", "P_average_RHS", "hamiltonian_average"):
+                self.writer.add("ehrenfest/" + name, np.asarray(getattr(qs, name), dtype=np.float64))
+
+    def spectrum(self):
+        qs = self.quant_sys
+        return hhg_spectrum(qs.P_average, qs.dt, qs.omega)
+
+
+def main(path="strong_field_physics.npz", max_frames=881):
+    with NpzWriter(path) as w:
+        run = StrongFieldRun(writer=w)
+        run.run(max_frames)
+        run.save_ehrenfest()
+    return run
+
+
+if __name__ == "__main__":
+    main()

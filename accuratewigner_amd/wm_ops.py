@@ -7,6 +7,7 @@ wigner2rho (:422-469) and rho2wigner_ (:471-501), each as four fused passes.
 import numpy as np
 
 from .engine import FWD, INV
+from .rtc import kconst
 
 SQRT2 = np.sqrt(2.0)
 
@@ -32,29 +33,32 @@ class WMGrid:
         self.X, self.P, self.XX, self.XX_prime, self.PP, self.PP_prime = X, P, XX, XX_prime, PP, PP_prime
         j = np.arange(Nx, dtype=np.float64)
         i = np.arange(Np, dtype=np.float64)
-        # coordinates as the reference kernels recompute them from their step constants
-        self.kXX = self.dXX * (j - 0.5 * Nx)
-        self.kXXp = self.dXX_prime * (i - 0.5 * Np)
-        self.kPP = self.dPP * (j - 0.5 * Nx)
-        self.kPPp = self.dPP_prime * (i - 0.5 * Np)
-        self.kX = self.dX * (j - 0.5 * Nx)
-        self.kP = self.dP * (i - 0.5 * Np)
+        # coordinates as the reference kernels recompute them from their #define'd steps
+        # (e.g. XX = dXX * (j - 0.5 * X_gridDIM), wigner_moyal_cuda_1d.py:769-770)
+        self.k_dXX, self.k_dXX_prime = kconst(self.dXX), kconst(self.dXX_prime)
+        self.k_dPP, self.k_dPP_prime = kconst(self.dPP), kconst(self.dPP_prime)
+        self.kXX = self.k_dXX * (j - 0.5 * Nx)
+        self.kXXp = self.k_dXX_prime * (i - 0.5 * Np)
+        self.kPP = self.k_dPP * (j - 0.5 * Nx)
+        self.kPPp = self.k_dPP_prime * (i - 0.5 * Np)
+        self.kX = kconst(self.dX) * (j - 0.5 * Nx)
+        self.kP = kconst(self.dP) * (i - 0.5 * Np)
 
 
 def shear_x(plan, g, data, dir_a, dir_b, conj=False, **kw):
     """phase_shearX (wigner_moyal_cuda_1d.py:639-654): exp(-i tan(pi/8) PP_j XX'_i) / Nx on rows
     (conj: the complex conjugate factor)."""
     c = -np.tan(np.pi / 8.)
-    plan.run(data, 1, dir_a, dir_b, chirp=-c if conj else c, lstep=g.dXX_prime, lcenter=0.5 * g.Np,
-             estep=g.dPP, ecenter=0.5 * g.Nx, scale=1.0 / g.Nx, **kw)
+    plan.run(data, 1, dir_a, dir_b, chirp=-c if conj else c, lstep=g.k_dXX_prime, lcenter=0.5 * g.Np,
+             estep=g.k_dPP, ecenter=0.5 * g.Nx, scale=1.0 / g.Nx, **kw)
 
 
 def shear_y(plan, g, data, dir_a, dir_b, conj=False, **kw):
     """phase_shearY (wigner_moyal_cuda_1d.py:656-670): exp(+i sin(pi/4) PP'_i XX_j) / Np on columns
     (conj: the complex conjugate factor)."""
     c = np.sin(np.pi / 4.)
-    plan.run(data, 0, dir_a, dir_b, chirp=-c if conj else c, lstep=g.dXX, lcenter=0.5 * g.Nx,
-             estep=g.dPP_prime, ecenter=0.5 * g.Np, scale=1.0 / g.Np, **kw)
+    plan.run(data, 0, dir_a, dir_b, chirp=-c if conj else c, lstep=g.k_dXX, lcenter=0.5 * g.Nx,
+             estep=g.k_dPP_prime, ecenter=0.5 * g.Np, scale=1.0 / g.Np, **kw)
 
 
 def wigner2rho(plan, g, W, rho):

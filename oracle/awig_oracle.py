@@ -72,6 +72,14 @@ def _as_grid(val, shape, dtype=np.float64):
     return np.broadcast_to(np.asarray(val, dtype=dtype), shape)
 
 
+def kconst(x):
+    """A float as a kernel sees it through ``#define name %.15e``
+    (wigner_moyal_cuda_1d.py:174-176, rho_vneumann_cuda_1d.py and
+    schrodinger_wigner_cuda_1d.py build their #define blocks the same way): 16 significant
+    digits.  The kernels recompute coordinates from these rounded steps."""
+    return float("%.15e" % float(x))
+
+
 def sign_matrix(rows, cols):
     """(-1)**(i + j) (wigner_moyal_cuda_1d.py:707-715)."""
     si = 1 - 2 * (np.arange(rows) % 2)
@@ -123,12 +131,12 @@ class WignerMoyalOracle:
         # :797-798, :823-824)
         j = np.arange(Nx, dtype=np.float64)
         i = np.arange(Np, dtype=np.float64)
-        self.kXX = self.dXX * (j - 0.5 * Nx)
-        self.kXXp = self.dXX_prime * (i - 0.5 * Np)
-        self.kPP = self.dPP * (j - 0.5 * Nx)
-        self.kPPp = self.dPP_prime * (i - 0.5 * Np)
-        self.kX = self.dX * (j - 0.5 * Nx)
-        self.kP = self.dP * (i - 0.5 * Np)
+        self.kXX = kconst(self.dXX) * (j - 0.5 * Nx)
+        self.kXXp = kconst(self.dXX_prime) * (i - 0.5 * Np)
+        self.kPP = kconst(self.dPP) * (j - 0.5 * Nx)
+        self.kPPp = kconst(self.dPP_prime) * (i - 0.5 * Np)
+        self.kX = kconst(self.dX) * (j - 0.5 * Nx)
+        self.kP = kconst(self.dP) * (i - 0.5 * Np)
         self.S = sign_matrix(Np, Nx)
 
         self.wignerfunction = np.zeros((Np, Nx), np.complex128)
@@ -476,8 +484,8 @@ class RhoVNeumannOracle:
         self.P_wigner = Pw[:, np.newaxis]
 
         j = np.arange(N, dtype=np.float64)
-        self.kX = self.dX * (j - 0.5 * N)   # :760-761
-        self.kP = self.dP * (j - 0.5 * N)   # :739-740
+        self.kX = kconst(self.dX) * (j - 0.5 * N)   # :760-761
+        self.kP = kconst(self.dP) * (j - 0.5 * N)   # :739-740
         self.S = sign_matrix(N, N)
         self.rho = np.zeros((N, N), np.complex128)
         self.wignerfunction = np.zeros((M, M), np.complex128)
@@ -596,8 +604,8 @@ class RhoVNeumannOracle:
         """phase_shearX / phase_shearY on the padded 2N grid (rho_vneumann_cuda_1d.py:608-639)."""
         M = self.XX_gridDIM
         k = np.arange(M, dtype=np.float64)
-        PP = self.dPP * (k - 0.5 * M)
-        XX = self.dX * (k - 0.5 * M)   # dXX = dX (:158)
+        PP = kconst(self.dPP) * (k - 0.5 * M)
+        XX = kconst(self.dX) * (k - 0.5 * M)   # dXX = dX (:158)
         a = np.tan(np.pi / 8.)
         SX = np.exp(1j * (-a * PP[None, :] * XX[:, None])) / float(M)
         b = -np.sin(np.pi / 4.)
@@ -749,8 +757,8 @@ class SchrodingerWignerOracle:
         self.P_wigner = self.P / SQRT2
         self.wigner_dXdP = self.dX * self.dP
         j = np.arange(N)
-        self.kX = self.dX * (j - 0.5 * N)                          # :821
-        self.kP = self.dP * (((j + N // 2) % N) - 0.5 * N)          # :803 (FFT order)
+        self.kX = kconst(self.dX) * (j - 0.5 * N)                          # :821
+        self.kP = kconst(self.dP) * (((j + N // 2) % N) - 0.5 * N)          # :803 (FFT order)
         self.wavefunction = np.zeros((1, N), np.complex128)
         self.isEhrenfest = diff_K is not None and diff_V is not None
         if self.isEhrenfest:

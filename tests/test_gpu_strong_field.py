@@ -1,0 +1,59 @@
+"""Strong-field driver (strong_field_physics_cuda_1d.py) on a reduced grid: frames and
+Ehrenfest lists match the oracle continued from the same ground state (FP64, 1e-10), the
+.npz archive keeps the reference's HDF5 layout, and the ground-state cache is reused."""
+import numpy as np
+import pytest
+
+from oracle.awig_oracle import WignerBlochOracle, kconst
+
+pytestmark = pytest.mark.gpu
+
+
+def _params():
+    from accuratewigner_amd.strong_field import sys_params
+    return sys_params(X_gridDIM=256, P_gridDIM=128, X_amplitude=40., P_amplitude=8., dt=0.05)
+
+
+def test_strong_field_frames_match_oracle(tmp_path):
+    from accuratewigner_amd.strong_field import NpzWriter, StrongFieldRun
+    p = _params()
+    gs = str(tmp_path / "gs.npy")
+    out = str(tmp_path / "sf.npz")
+    with NpzWriter(out) as w:
+        run = StrongFieldRun(p, writer=w, ground_state_file=gs, num_iteration=20, ground_state_max_steps=200)
+        assert not run.ground_state_loaded
+        run.run(max_frames=3)
+        run.save_ehrenfest()
+    qs = run.quant_sys
+    F, om, dt = p["F"], p["omega"], p["dt"]
+    E = lambda t: -F * np.sin(om * t) * np.sin(om * t / 16.) ** 2    # noqa: E731
+    # the mask sees XX_max, XX_prime_max as #define'd values
+    XXm, XXpm = kconst(qs.XX.max()), kconst(qs.XX_prime.max())
+    o = WignerBlochOracle(
+        X_gridDIM=p["X_gridDIM"], X_amplitude=p["X_amplitude"], P_gridDIM=p["P_gridDIM"],
+        P_amplitude=p["P_amplitude"], dt=dt, dbeta=dt,
+        V=lambda X, t: -1. / np.sqrt(X * X + 1.37) + X * E(t), K=lambda P, t: 0.5 * P * P,
+        diff_K=lambda X, P, t: P, diff_V=lambda X, P, t: X / (X * X + 1.37) ** 1.5 + E(t),
+        abs_boundary_x=lambda XX, XXp: np.abs(np.sin(0.5 * np.pi * (XX + XXm) / XXm)
+                                              * np.sin(0.5 * np.pi * (XXp + XXpm) / XXpm)) ** (dt * 0.05))
+    o.rho = np.load(gs).astype(np.complex128)
+    o.t = np.float64(0.)
+    z = np.load(out, allow_pickle=False)
+    for f in range(3):
+        o.propagate(20)
+        Wo = o.wignerfunction.real
+        Wg = z["frames/%d/wigner" % f]
+        assert np.linalg.norm(Wg - Wo) / np.linalg.norm(Wo) < 1e-10, f
+        assert abs(float(z["frames/%d/t" % f]) - o.t) < 1e-12
+        assert np.max(np.abs(z["frames/%d/prob" % f] - o.rho.diagonal())) < 1e-10 * np.max(np.abs(o.rho))
+    for name in ("X_average", "P_average", "X_average_RHS", "P_average_RHS", "hamiltonian_average"):
+        a, b = z["ehrenfest/" + name], np.array(getattr(o, name))
+        assert a.shape == (60,)
+        assert np.max(np.abs(a - b)) < 1e-10 * max(1.0, np.max(np.abs(b))), name
+    assert float(z["settings/dt"]) == dt and str(z["settings/V"]) == p["V"]
+    assert z["settings/X_wigner"].shape == (1, 256) and z["settings/P_wigner"].shape == (128, 1)
+    assert "settings/E" not in z.files          # Python functions are skipped, as h5py does
+    # the cached ground state is loaded by the next run
+    run2 = StrongFieldRun(p, ground_state_file=gs, num_iteration=20)
+    assert run2.ground_state_loaded
+    assert np.array_equal(run2.quant_sys.rho.get(), np.load(gs))

@@ -3,7 +3,7 @@ oracle restatement of wigner_moyal_cuda_1d.py."""
 import numpy as np
 import pytest
 
-from oracle.awig_oracle import WignerMoyalOracle
+from oracle.awig_oracle import WignerMoyalOracle, kconst
 
 pytestmark = pytest.mark.gpu
 
@@ -97,7 +97,7 @@ def test_wm_time_dependent_and_masks():
     Vf = lambda X, t: 0.5 * X * X + X * 0.1 * np.sin(2. * t)  # noqa: E731
     ax = "pow(abs(sin(0.5 * M_PI * (XX + XX_max) / XX_max) * sin(0.5 * M_PI * (XX_prime + XX_prime_max) / XX_prime_max)), dt * 0.05)"
     g, o = _pair(V=V, Vf=Vf, abs_boundary_x=ax)
-    XXm, XXpm = g.XX.max(), g.XX_prime.max()
+    XXm, XXpm = kconst(g.XX.max()), kconst(g.XX_prime.max())
     o.abs_boundary_x = lambda XX, XXp: np.abs(np.sin(0.5 * np.pi * (XX + XXm) / XXm)
                                               * np.sin(0.5 * np.pi * (XXp + XXpm) / XXpm)) ** (0.01 * 0.05)
     assert g._masks[0][0] == "vec"
