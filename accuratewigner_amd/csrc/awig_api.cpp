@@ -18,6 +18,7 @@ struct aw_plan {
     void *tw_cols = nullptr;   // roots for FFTs of length cols (axis 1)
     double *partials = nullptr;
     int red_blocks = 0;
+    double *diag_work = nullptr;   // cyclic-diagonal partial sums, allocated on first use
 };
 
 struct aw_module {
@@ -114,6 +115,7 @@ int aw_plan_destroy(aw_plan *p) {
     if (p->tw_rows) (void)hipFree(p->tw_rows);
     if (p->tw_cols) (void)hipFree(p->tw_cols);
     if (p->partials) (void)hipFree(p->partials);
+    if (p->diag_work) (void)hipFree(p->diag_work);
     delete p;
     return AW_OK;
 }
@@ -260,6 +262,19 @@ int aw_pair_dot(aw_plan *p, const void *a, const void *b, const double *u, const
     if (!a || !b || !u || !v || !out) return set_error(AW_ERR_INVALID_VALUE, "NULL pair-dot argument");
     return launch_pair_dot(p->prec, a, b, (long long)p->rows * p->cols, p->cols, u, v, p->batch, p->partials,
                            p->red_blocks, out, (hipStream_t)stream);
+}
+
+int aw_cyclic_diag_sums(aw_plan *p, const void *data, double *out, aw_stream stream) {
+    if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
+    if (!data || !out) return set_error(AW_ERR_INVALID_VALUE, "NULL data or out");
+    if (p->rows != p->cols || p->rows < 64)
+        return set_error(AW_ERR_INVALID_SIZE, "cyclic diagonal sums need a square plan with n >= 64");
+    const int n = p->rows;
+    const int nblk = n / kDiagRowsPerBlock;
+    if (!p->diag_work) {
+        AW_HIP(hipMalloc((void **)&p->diag_work, sizeof(double) * 2 * (size_t)n * nblk * p->batch));
+    }
+    return launch_cyclic_diag(p->prec, data, n, p->batch, p->diag_work, out, (hipStream_t)stream);
 }
 
 int aw_sum_partials(const double *partials, int nblk, int ncomp, int batch, double *out,

@@ -203,6 +203,30 @@ int launch_pair_dot(int prec, const void *a, const void *b, long long n_item, in
     return AW_OK;
 }
 
+// signed cyclic diagonal sums of square items: out[b][d] = (-1)^d sum_i z[i][(i + d) mod n]
+// (complex FP64).  For fixed i the threads d, d+1, ... read consecutive columns of row i.
+template <typename R>
+__global__ __launch_bounds__(256) void k_cyclic_diag(const void *data, int n, double *work) {
+    using T2 = typename Cx<R>::T;
+    const int d = blockIdx.x * 256 + threadIdx.x;
+    const int r0 = blockIdx.y * kDiagRowsPerBlock;
+    const int b = blockIdx.z;
+    const T2 *z = reinterpret_cast<const T2 *>(data) + (long long)b * n * n;
+    double re = 0.0, im = 0.0;
+    if (d < n) {
+#pragma unroll 4
+        for (int i = r0; i < r0 + kDiagRowsPerBlock; ++i) {
+            const T2 q = z[(long long)i * n + ((i + d) & (n - 1))];
+            re += q.x;
+            im += q.y;
+        }
+        const double s = (d & 1) ? -1.0 : 1.0;
+        double *w = work + ((long long)b * gridDim.y + blockIdx.y) * 2 * n;
+        w[2 * d] = s * re;
+        w[2 * d + 1] = s * im;
+    }
+}
+
 __global__ void k_sum_partials(const double *partials, int nblk, int ncomp, double *out) {
     const int b = blockIdx.x;
     for (int c = threadIdx.x; c < ncomp; c += blockDim.x) {
@@ -223,6 +247,19 @@ int launch_stats(int prec, const void *data, long long n_item, int rows, int col
     hipLaunchKernelGGL(k_sum_partials, dim3(batch), dim3(64), 0, st, partials, nblk, NSTAT, out);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_stats: %s", hipGetErrorString(e));
+    return AW_OK;
+}
+
+int launch_cyclic_diag(int prec, const void *data, int n, int batch, double *work, double *out, hipStream_t st) {
+    const dim3 grid((n + 255) / 256, n / kDiagRowsPerBlock, batch);
+    if (prec == AW_PREC_F64)
+        hipLaunchKernelGGL(k_cyclic_diag<double>, grid, dim3(256), 0, st, data, n, work);
+    else
+        hipLaunchKernelGGL(k_cyclic_diag<float>, grid, dim3(256), 0, st, data, n, work);
+    // second stage: out[b][c] = sum over the row blocks (fixed order), 2 n doubles per item
+    hipLaunchKernelGGL(k_sum_partials, dim3(batch), dim3(256), 0, st, work, n / kDiagRowsPerBlock, 2 * n, out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_cyclic_diag: %s", hipGetErrorString(e));
     return AW_OK;
 }
 

@@ -126,6 +126,12 @@ class Plan:
                   ctypes.c_void_p(current_stream()))
         return out
 
+    def cyclic_diag_sums(self, data, out):
+        """out[batch, n] complex128 (device): (-1)^d sum_i data[i, (i + d) mod n]."""
+        _lib.call("aw_cyclic_diag_sums", self.handle, ctypes.c_void_p(_p(data)), ctypes.c_void_p(_p(out)),
+                  ctypes.c_void_p(current_stream()))
+        return out
+
     def scale_from_stats(self, stats_ptr, stride, comp, mode, factor, dscale, batch=None):
         _lib.call("aw_scale_from_stats", ctypes.c_void_p(stats_ptr), stride, comp, mode, factor,
                   ctypes.c_void_p(_p(dscale)), self.batch if batch is None else batch,

@@ -97,6 +97,7 @@ class RhoVNeumannCUDA1D:
         self.plan = Plan(N, N, 1, self.precision)
         self.state_dtype = self.plan.state_dtype
         self._wplan = None
+        self._cplan = None      # 1 x N FP64 plan for momentum-diagonal transforms
         self._mod = rtc.Module(self._physics_source(), "rho_vneumann_physics.hip")
         self._dev_kX = to_device(self._kX)
         self._dev_kP = to_device(self._kP)
@@ -334,8 +335,11 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
         exprs_p = [self.diff_K, "P", self.K]
         xw = empty((len(exprs_x), N), np.float64)
         pw = empty((len(exprs_p), N), np.float64)
-        diag = empty((N,), self.state_dtype)
-        dstats = empty((1, 8), np.float64)
+        csum = empty((N,), np.complex128)
+        chat = empty((N,), np.complex128)
+        if self._cplan is None:
+            self._cplan = Plan(1, N, 1, "fp64")
+        pmod = self._obs_module(["X"])
 
         def observe(s):
             t = float(ts[s])
@@ -343,12 +347,13 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
             self._weights(exprs_x, t, xw)
             self._weights(exprs_p, t, pw)
             self._diag_obs(xw, len(exprs_x), ring.ptr + s * 64)
-            for k in range(len(exprs_p)):
-                self.plan.run(self.rho, 1, FWD, INV, sign_in=1, sign_out=1,
-                              elem_fac=self._weight_vec(pw, k, N), scale=1.0 / N,
-                              out=diag, out_mode=1)
-                reduce_vec(self.plan.prec, diag, N, 1, dstats)
-                self._copy_scalar(dstats, 0, ring.ptr + (s * 8 + 4 + k) * 8)
+            # p-observables sum_j [S I1 (g/N) F1 S rho]_jj = Re sum_k g(P_k) F(c)_k / N with
+            # c the signed cyclic diagonal sums of rho: one read of rho for all three
+            self.plan.cyclic_diag_sums(self.rho, csum)
+            self._cplan.fft(csum, 1, FWD, out=chat)
+            pmod.launch("aw_p_obs", (1,), (256,), [chat.ptr, pw.ptr, ctypes.c_int(len(exprs_p)),
+                                                   ctypes.c_int(N), ring.ptr + (s * 8 + 4) * 8],
+                        current_stream())
             # rho /= Re tr(rho) dX before the next step (applied by its opening pass)
             self.plan.scale_from_stats(ring.ptr + s * 64, 8, 0, 0, self.dX, self._dscale)
         for c0 in range(0, steps, CHUNK_STEPS):
@@ -405,6 +410,28 @@ extern "C" __global__ void aw_diag_obs(const %s *rho, const double *w, int nobs,
     }
 }
 extern "C" __global__ void aw_copy_scalar(const double *src, double *dst) { *dst = *src; }
+extern "C" __global__ void aw_p_obs(const double2 *chat, const double *w, int nobs, int n, double *out)
+{
+    // out[q] = sum_k w_q[k] Re chat[k] / n (one block): momentum observables from the
+    // DFT of the signed cyclic diagonal sums of rho
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const double c = chat[k].x;
+        for (int q = 0; q < nobs && q < 8; ++q) acc[q] += c * w[q * n + k];
+    }
+    __shared__ double red[4][8];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int c = 0; c < 8; ++c) {
+        const double s = aw_wave_sum(acc[c]);
+        if (lane == 0) red[wid][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8 && threadIdx.x < nobs) {
+        double s = 0.;
+        for (int q = 0; q < 4; ++q) s += red[q][threadIdx.x];
+        out[threadIdx.x] = s / n;
+    }
+}
 """ % (body, self._state_t()))
             mod = rtc.Module(src, "rho_vneumann_obs.hip")
             self._obs_cache[key] = mod

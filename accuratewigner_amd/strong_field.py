@@ -52,6 +52,47 @@ def sys_params(X_gridDIM=2 * 1024, P_gridDIM=512, X_amplitude=150., P_amplitude=
     )
 
 
+def rv_sys_params(X_gridDIM=1024, X_amplitude=200., dt=0.02, F=0.05, omega=0.06, cycles=4):
+    """Von Neumann rho(x, x') in a laser pulse (the RhoVNeumannCUDA1D form of the driver):
+    soft-core V = -1/sqrt(X^2 + 1) + X E(t), E(t) = F sin^2(pi t / T) cos(omega t) over
+    T = cycles 2 pi / omega, and an absorbing edge over the outer 10% of the box,
+    b(X) = cos(pi/2 (|X| - 0.9 X_amplitude) / (0.1 X_amplitude))^(1/8) (1 inside)."""
+    return dict(
+        t=0.,
+        dt=dt,
+        X_gridDIM=X_gridDIM,
+        X_amplitude=X_amplitude,
+        F=F,
+        omega=omega,
+        T_pulse=cycles * 2. * np.pi / omega,
+        functions="""
+    __device__ double E(double t)
+    {
+        const double s = sin(M_PI * t / T_pulse);
+        return F * s * s * cos(omega * t);
+    }
+    """,
+        abs_boundary_x="(fabs(X) < 0.9 * X_amplitude ? 1. : "
+                       "pow(cos(0.5 * M_PI * (fabs(X) - 0.9 * X_amplitude) / (0.1 * X_amplitude)), 0.125))",
+        K="0.5 * P * P",
+        diff_K="P",
+        V="-1. / sqrt(X * X + 1.) + X * E(t)",
+        diff_V="X / pow(X * X + 1., 1.5) + E(t)",
+    )
+
+
+def rv_ground_state_rho(params, nsteps=5000):
+    """rho0 = |psi0><psi0| of the field-free soft-core atom: SchrodingerWignerCUDA1D imaginary
+    time (set_ground_state, schrodinger_wigner_cuda_1d.py:484-503) on the same grid, then
+    rho[i, j] = psi(X_j) conj(psi(X_i)) (bra on i, ket on j)."""
+    from .schrodinger_wigner_cuda_1d import SchrodingerWignerCUDA1D
+    sw = SchrodingerWignerCUDA1D(X_gridDIM=params["X_gridDIM"], X_amplitude=params["X_amplitude"],
+                                 dt=params["dt"], t=0., V="-1. / sqrt(X * X + 1.)", K="0.5 * P * P")
+    sw.set_ground_state(nsteps)
+    psi = sw.wavefunction.get().reshape(-1)
+    return np.outer(np.conj(psi), psi)
+
+
 class NpzWriter:
     """Incremental ``.npz`` writer: every ``add`` appends one ``<key>.npy`` member."""
 

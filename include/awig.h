@@ -164,6 +164,16 @@ int aw_pair_dot(aw_plan *plan, const void *a, const void *b, const double *u, co
                 double *out, aw_stream stream);
 
 /*
+ * Signed cyclic diagonal sums of square items (n >= 64), FP64 complex:
+ *   out[b][d] = (-1)^d sum_i data[b][i][(i + d) mod n],  d = 0..n-1   (2 n doubles per item).
+ * The forward DFT of this vector is the momentum-space diagonal of rho scaled by n, so a
+ * momentum observable sum_j [S I1 (g(P_j)/n) F1 S rho]_jj (rho_vneumann_cuda_1d.py:551-560,
+ * :676-684, :811-821) is Re sum_k g(P_k) F(out)_k / n: one read of rho for any number of
+ * g.  The plan allocates its scratch (2 n^2 / 16 doubles per item) on first use.
+ */
+int aw_cyclic_diag_sums(aw_plan *plan, const void *data, double *out, aw_stream stream);
+
+/*
  * out[b][c] = sum_i partials[b][i][c] for i < nblk, c < ncomp, summed in fixed order (the
  * second stage of every block-partial reduction, including the hipRTC observable kernels).
  */
