@@ -12,18 +12,21 @@ template <typename R, int N, bool STRIDED, int LPB, int TWM> constexpr int lds_b
 }
 
 // Per (precision, line length, line kind):
-//  * E = 16 elements per thread (8 below 256), so a 1024-point line is 64 lanes, radix 16-16-4.
+//  * E = 16 elements per thread (8 below 256), so a 1024-point line is 64 lanes, radix 16-16-4;
+//    FP32 columns of 512 and 1024 points use E = 32 (radix 32-16 / 32-32: one LDS exchange,
+//    8 columns per 128 / 256-thread block; 14% faster at 1024^2 x 96 than E = 16);
 //  * rows: 4 lines per 256-thread block in FP32, 2 lines per 128-thread block in FP64;
 //  * columns: 8 adjacent columns per block (64-byte row segments in FP32, 128 in FP64);
 //  * twiddles in LDS (FP32 always, FP64 columns when they fit) or read through the cache;
 //  * the largest power-of-two LPB that fits the 160 KB LDS.
 // Measured at 1024^2 x 48 (FP32) / x 24 (FP64) states on MI355X.
 template <typename R, int N, bool STRIDED> struct PassTune {
-    static constexpr int E = (N >= 256) ? 16 : 8;
-    static constexpr int T = N / E;
     static constexpr bool F32 = sizeof(R) == 4;
+    static constexpr bool WIDE = F32 && STRIDED && (N == 512 || N == 1024);
+    static constexpr int E = WIDE ? 32 : ((N >= 256) ? 16 : 8);
+    static constexpr int T = N / E;
     static constexpr int WANT_ROWS = F32 ? (256 / T > 1 ? 256 / T : 1) : (128 / T > 1 ? 128 / T : 1);
-    static constexpr int WANT_COLS = (8 > 256 / T) ? 8 : 256 / T;
+    static constexpr int WANT_COLS = WIDE ? 8 : ((8 > 256 / T) ? 8 : 256 / T);
     static constexpr int WANT0 = STRIDED ? WANT_COLS : WANT_ROWS;
     // at most 512 threads per block (a 1024-thread block caps a thread at 128 VGPRs)
     static constexpr int WANT = (WANT0 * T > 512) ? (512 / T > 1 ? 512 / T : 1) : WANT0;

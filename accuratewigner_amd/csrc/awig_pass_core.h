@@ -228,11 +228,12 @@ __device__ __forceinline__ void fft_line(int dir, T2 *v, T2 *L, int l, int u, co
 // passes (GEN = 0) only multiply by line x element factor vectors (and an optional mask),
 // which keeps their code to two FFT bodies.
 template <typename R_, int N_, int E_, int LPB_, bool STRIDED_, int TWM_, int DA_ = 2, int DB_ = 2,
-          int GEN_ = 1, int MINW_ = 1>
+          int GEN_ = 1, int MINW_ = 1, int PF_ = 0>
 struct KCfg {
     using R = R_;
     static constexpr int N = N_, E = E_, LPB = LPB_, TWM = TWM_, DA = DA_, DB = DB_, GEN = GEN_;
     static constexpr int MINW = MINW_;   // minimum waves per SIMD the register budget must allow
+    static constexpr int PF = PF_;       // 1: load the factor vectors with the data, before the first FFT
     static constexpr bool STRIDED = STRIDED_;
     static constexpr int T = N / E;
     static constexpr int THREADS = LPB * T;
@@ -300,6 +301,18 @@ void k_line_pass(PassArgs a) {
     T2 v[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) v[k] = src[k * kstep];
+    // optional prefetch of the element factors: their load latency then overlaps the data
+    // loads instead of stalling between the two FFTs (costs 2E registers)
+    T2 efp[C::PF ? E : 1];
+    T2 lfp = mk<T2>(1.0, 0.0);
+    if constexpr (C::PF) {
+        if (a.has_factor && a.elem_fac) {
+            const T2 *ef = reinterpret_cast<const T2 *>(a.elem_fac) + b * a.ef_bstride + u;
+#pragma unroll
+            for (int k = 0; k < E; ++k) efp[k] = ef[T * k];
+        }
+        if (a.has_factor && a.line_fac) lfp = reinterpret_cast<const T2 *>(a.line_fac)[b * a.lf_bstride + line];
+    }
 
     if (C::GEN && a.sign_in) {
 #pragma unroll
@@ -325,13 +338,13 @@ void k_line_pass(PassArgs a) {
         }
         T2 lf = mk<T2>(lc.x, lc.y);
         if (a.line_fac) {
-            const T2 q = reinterpret_cast<const T2 *>(a.line_fac)[b * a.lf_bstride + line];
+            const T2 q = C::PF ? lfp : reinterpret_cast<const T2 *>(a.line_fac)[b * a.lf_bstride + line];
             lf = cmul(lf, q);
         }
         if (a.elem_fac) {
             const T2 *ef = reinterpret_cast<const T2 *>(a.elem_fac) + b * a.ef_bstride + u;
 #pragma unroll
-            for (int k = 0; k < E; ++k) v[k] = cmul(v[k], cmul(lf, ef[T * k]));
+            for (int k = 0; k < E; ++k) v[k] = cmul(v[k], cmul(lf, C::PF ? efp[k] : ef[T * k]));
         } else {
 #pragma unroll
             for (int k = 0; k < E; ++k) v[k] = cmul(v[k], lf);

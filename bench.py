@@ -76,6 +76,50 @@ def cpu_baseline(N, seconds, workers=None):
     return N * N * n / el, n, el
 
 
+def copy_bandwidth(mib=1024, reps=10):
+    """Achievable HBM bandwidth of a plain copy (libawig's k_convert, complex128 -> complex128)
+    of ``mib`` MiB, timed with HIP events on the launch stream: the practical ceiling next to
+    the 8 TB/s peak, and the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    from accuratewigner_amd import engine
+    n = mib * 2 ** 20 // 16
+    a = torch.ones(2 * n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    engine.convert(a.data_ptr(), 0, b.data_ptr(), 0, n)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        engine.convert(a.data_ptr(), 0, b.data_ptr(), 0, n)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return 2 * n * 16 / (ms * 1e-3) / 1e9, ms
+
+
+def serving_run(n_jobs, seed=11, chunk=64):
+    """Serving workload: a seeded Poisson stream (mean 200 req/s) of WM / WB jobs, grids from
+    {128, 256, 512, 1024}^2, V = X^2/2 + c4 X^4 with c4 ~ U(0, 0.2), beta ~ U(0.5, 5),
+    steps ~ U{200..5000}, FP32 / FP64 tiers, served in real time by continuous batching."""
+    from accuratewigner_amd.serving import Job, Server, poisson_jobs
+    srv = Server(hbm_budget_bytes=32 << 30, chunk=chunk, max_slots=64)
+    # warm-up: one short job per pool key (hipRTC compiles, plans, pool allocations)
+    warm = [Job(k, N, prec, steps=8, beta=0.04, job_id=-1) for k in ("wm", "wb")
+            for N in srv.grids for prec in ("mixed", "fp64")]
+    srv.run(warm, realtime=False)
+    jobs = poisson_jobs(n_jobs, rate=200.0, seed=seed)
+    srv.stats.clear()
+    done = srv.run(jobs, realtime=True)
+    lat = np.array([j.latency for j in done]) * 1e3
+    work = float(sum(j.N * j.N * j.steps for j in done))
+    wall = srv.stats["wall_s"]
+    return {"workload": "Poisson stream, mean 200 req/s, %d jobs (seed %d), WM/WB, grids 128..1024, "
+                        "FP32/FP64 tiers, steps 200..5000, beta 0.5..5, chunk %d" % (n_jobs, seed, chunk),
+            "jobs": len(done), "value": work / wall, "unit": "gp-steps/s", "wall_s": wall,
+            "arrival_span_s": float(jobs[-1].arrival), "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +130,9 @@ def main():
     ap.add_argument("--fp64-batch", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=200)
+    ap.add_argument("--copy-only", type=int, default=0, metavar="MIB",
+                    help="only run the copy-bandwidth kernel on MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -94,6 +141,11 @@ def main():
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     N, B, K, W = args.grid, args.batch, args.steps, args.warmup
+    if args.copy_only:
+        gbps, ms = copy_bandwidth(args.copy_only, reps=3)
+        print(json.dumps({"copy_MiB": args.copy_only, "copy_GBps": gbps, "ms": ms}), flush=True)
+        return
+    copy_gbps, _ = copy_bandwidth()
 
     # -- headline: mixed precision, batch B -------------------------------------------------
     mixed = make_system(N, B, "mixed")
@@ -131,6 +183,8 @@ def main():
         lat.append((time.perf_counter() - t0) * 1e3)
     lat = np.array(lat[2:])
 
+    serving = serving_run(args.serve_jobs) if args.serve_jobs > 0 else None
+
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_wm1024_mixed_b%d.json" % B)
     if os.path.exists(tf):
@@ -161,9 +215,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "kernel": "k_line_pass (row and column passes)", "avg_pass_ms": pass_ms,
-                     "bytes_per_pass": bytes_per_pass},
+                     "bytes_per_pass": bytes_per_pass, "copy_GBps": copy_gbps,
+                     "frac_of_copy": achieved / copy_gbps},
         "latency": {"workload": "one 256x256 WM state, 1000 steps, mixed", "p50_ms": float(np.percentile(lat, 50)),
                     "p99_ms": float(np.percentile(lat, 99))},
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
