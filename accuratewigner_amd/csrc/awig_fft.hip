@@ -250,6 +250,17 @@ int launch_stats(int prec, const void *data, long long n_item, int rows, int col
     return AW_OK;
 }
 
+// second stage for many components: one thread per (item, component), fixed summation order
+__global__ void k_sum_partials_wide(const double *partials, int nblk, int ncomp, double *out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (c >= ncomp) return;
+    const double *p = partials + (long long)b * nblk * ncomp + c;
+    double s = 0.0;
+    for (int i = 0; i < nblk; ++i) s += p[(long long)i * ncomp];
+    out[(long long)b * ncomp + c] = s;
+}
+
 int launch_cyclic_diag(int prec, const void *data, int n, int batch, double *work, double *out, hipStream_t st) {
     const dim3 grid((n + 255) / 256, n / kDiagRowsPerBlock, batch);
     if (prec == AW_PREC_F64)
@@ -257,7 +268,8 @@ int launch_cyclic_diag(int prec, const void *data, int n, int batch, double *wor
     else
         hipLaunchKernelGGL(k_cyclic_diag<float>, grid, dim3(256), 0, st, data, n, work);
     // second stage: out[b][c] = sum over the row blocks (fixed order), 2 n doubles per item
-    hipLaunchKernelGGL(k_sum_partials, dim3(batch), dim3(256), 0, st, work, n / kDiagRowsPerBlock, 2 * n, out);
+    hipLaunchKernelGGL(k_sum_partials_wide, dim3((2 * n + 255) / 256, batch), dim3(256), 0, st, work,
+                       n / kDiagRowsPerBlock, 2 * n, out);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_cyclic_diag: %s", hipGetErrorString(e));
     return AW_OK;

@@ -1,0 +1,42 @@
+"""hipGraph capture of fixed libawig launch sequences.
+
+A propagation loop that records observables every step issues about ten small launches per
+step; at 256^2 the host then paces the GPU.  Such a loop is run in chunks whose launches
+read every per-chunk input (factor tables, times, the output ring) from buffers at fixed
+addresses, so one chunk can be captured once (torch.cuda.CUDAGraph over the current HIP
+stream, which every libawig launch uses) and replayed for every later chunk after its
+inputs are rewritten.  Captured sequences contain kernel launches only: no allocation,
+memset or host synchronisation.
+
+Set AWIG_GRAPHS=0 to run every chunk eagerly (same kernels, same order, same results).
+"""
+import os
+
+
+def enabled():
+    return os.environ.get("AWIG_GRAPHS", "1") != "0"
+
+
+class Captured:
+    """One captured launch sequence; ``replay()`` runs it on the current stream."""
+
+    def __init__(self, fn):
+        import torch
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            fn()
+
+    def replay(self):
+        self.graph.replay()
+
+
+def run_chunk(cache, key, fn, full):
+    """Run ``fn`` (a fixed launch sequence) through a cached graph when ``full`` (the chunk
+    has the captured length) and graphs are enabled, else eagerly."""
+    if not (full and enabled()):
+        fn()
+        return
+    g = cache.get(key)
+    if g is None:
+        g = cache[key] = Captured(fn)
+    g.replay()

@@ -13,7 +13,7 @@ from types import FunctionType, MethodType
 
 import numpy as np
 
-from . import rtc
+from . import graphs, rtc
 from .device import DeviceArray, current_stream, empty, require_gpu, to_device
 from .engine import FWD, INV, Plan, mem_info, reduce_vec, sum_partials
 
@@ -98,6 +98,7 @@ class RhoVNeumannCUDA1D:
         self.state_dtype = self.plan.state_dtype
         self._wplan = None
         self._cplan = None      # 1 x N FP64 plan for momentum-diagonal transforms
+        self._ehr = None        # persistent chunk buffers and captured graphs of the Ehrenfest loop
         self._mod = rtc.Module(self._physics_source(), "rho_vneumann_physics.hip")
         self._dev_kX = to_device(self._kX)
         self._dev_kP = to_device(self._kP)
@@ -231,28 +232,37 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
             ts[k] = t
         return ts
 
-    def _chunk_factors(self, ts, merged=True):
+    def _chunk_factors(self, ts, merged=True, bufs=None):
         """FP64 phase tables (hipRTC) and factor vectors for steps at times ts.  merged: the
         closing half-kick of a step and the opening one of the next share one column pass
         (n + 1 column vectors); otherwise every step has its own opening and closing
-        vectors (2 n column vectors, open at index 2 s, close at 2 s + 1)."""
+        vectors (2 n column vectors, open at index 2 s, close at 2 s + 1).  bufs: write the
+        times, tables and (unmerged) factors into these preallocated arrays."""
         n = len(ts)
         N, dt, p = self.X_gridDIM, self.dt, self.plan
-        ts_dev = to_device(ts)
-        Vt = empty((n, N), np.float64)
-        Kt = empty((n, N), np.float64)
+        if bufs is None:
+            ts_dev = to_device(ts)
+            Vt = empty((n, N), np.float64)
+            Kt = empty((n, N), np.float64)
+        else:
+            ts_dev, Vt, Kt = bufs["ts"], bufs["Vt"], bufs["Kt"]
+            ts_dev[:n] = ts
         self._eval_1d(0, self._dev_kX, N, ts_dev, n, 0.5 * dt, Vt)
         self._eval_1d(1, self._dev_kP, N, ts_dev, n, 0.5 * dt, Kt)
         bx = self._bx.ptr if self._bx is not None else None
         bp = self._bp.ptr if self._bp is not None else None
-        rowL = p.new_factors(N, n)
-        rowE = p.new_factors(N, n)
+        if bufs is None:
+            rowL = p.new_factors(N, n)
+            rowE = p.new_factors(N, n)
+        else:
+            colL, colE, rowL, rowE = bufs["fac"]
         # expK (:733-745): line i -> exp(+i dt K(P_i)) bp_i / N, elem j -> exp(-i dt K(P_j)) bp_j
         p.factors(rowL, N, n, A=Kt.ptr, ca=dt, mag=bp, scale=1.0 / N)
         p.factors(rowE, N, n, A=Kt.ptr, ca=-dt, mag=bp)
         if not merged:
-            colL = p.new_factors(N, 2 * n)
-            colE = p.new_factors(N, 2 * n)
+            if bufs is None:
+                colL = p.new_factors(N, 2 * n)
+                colE = p.new_factors(N, 2 * n)
             for v, scale in ((0, 1.0), (1, 1.0 / N)):
                 # expV (:754-770) with S = (-1)^(i+j) split as (-1)^j (-1)^i
                 p.factors(colL, N, n, A=Vt.ptr, ca=-0.5 * dt, mag=bx, alt_sign=True,
@@ -291,21 +301,29 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
             e1.record()
             ev.append((e0, e1, 2 * n + 1))
 
-    def _run_steps_unmerged(self, ts, observe):
-        """Every step as [EV, I0] [F1, EK, I1] [F0, EV] so that rho is complete after it;
-        ``observe(s)`` runs after step s and leaves a pending device scale for step s + 1."""
-        n = len(ts)
+    def _ehrenfest_chunk(self, n):
+        """n steps [EV, I0] [F1, EK, I1] [F0, EV] from the chunk buffers (applying the pending
+        normalisation), each followed by the trace and x-observables from diag rho, the
+        p-observables from the signed cyclic diagonal sums (one read of rho for all three:
+        sum_j [S I1 (g/N) F1 S rho]_jj = Re sum_k g(P_k) F(c)_k / N) and the next scale
+        1 / (Re tr(rho) dX).  Only launches, from fixed addresses."""
+        bufs, p, rho = self._ehr, self.plan, self.rho
         N = self.X_gridDIM
         csz = np.dtype(self.state_dtype).itemsize
-        colL, colE, rowL, rowE = self._chunk_factors(ts, merged=False)
-        p, rho = self.plan, self.rho
-        for s in range(n):
-            # one step = three passes; its opening / closing vectors sit at 2 s and 2 s + 1
-            p.steps(rho, 1, colL.ptr + 2 * s * N * csz, colE.ptr + 2 * s * N * csz,
-                    rowL.ptr + s * N * csz, rowE.ptr + s * N * csz,
-                    dscale=self._dscale if self._scale_pending else None)
+        colL, colE, rowL, rowE = bufs["fac"]
+        stage, xw, pw = bufs["stage"], bufs["xw"], bufs["pw"]
+        pmod = self._obs_module(["X"])
+        for k in range(n):
+            p.steps(rho, 1, colL.ptr + 2 * k * N * csz, colE.ptr + 2 * k * N * csz,
+                    rowL.ptr + k * N * csz, rowE.ptr + k * N * csz, dscale=self._dscale)
             self._scale_pending = False
-            observe(s)
+            self._diag_obs_ptr(xw.ptr + k * 3 * N * 8, 3, stage.ptr + k * 64)
+            p.cyclic_diag_sums(rho, bufs["csum"])
+            self._cplan.fft(bufs["csum"], 1, FWD, out=bufs["chat"])
+            pmod.launch("aw_p_obs", (1,), (256,), [bufs["chat"].ptr, pw.ptr + k * 3 * N * 8, ctypes.c_int(3),
+                                                   ctypes.c_int(N), stage.ptr + (k * 8 + 4) * 8],
+                        current_stream())
+            p.scale_from_stats(stage.ptr + k * 64, 8, 0, 0, self.dX, self._dscale)
             self._scale_pending = True
 
     def single_step_propagation(self):
@@ -330,34 +348,41 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
             return self
         self._flush_scale()
         N = self.X_gridDIM
+        CH = CHUNK_STEPS
         ring = empty((steps, 8), np.float64)
         exprs_x = ["X", self.diff_V, self.V]
         exprs_p = [self.diff_K, "P", self.K]
-        xw = empty((len(exprs_x), N), np.float64)
-        pw = empty((len(exprs_p), N), np.float64)
-        csum = empty((N,), np.complex128)
-        chat = empty((N,), np.complex128)
-        if self._cplan is None:
+        if self._ehr is None:
+            p = self.plan
             self._cplan = Plan(1, N, 1, "fp64")
-        pmod = self._obs_module(["X"])
-
-        def observe(s):
-            t = float(ts[s])
-            # trace of rho (normalisation) and the x-observables from the diagonal
-            self._weights(exprs_x, t, xw)
-            self._weights(exprs_p, t, pw)
-            self._diag_obs(xw, len(exprs_x), ring.ptr + s * 64)
-            # p-observables sum_j [S I1 (g/N) F1 S rho]_jj = Re sum_k g(P_k) F(c)_k / N with
-            # c the signed cyclic diagonal sums of rho: one read of rho for all three
-            self.plan.cyclic_diag_sums(self.rho, csum)
-            self._cplan.fft(csum, 1, FWD, out=chat)
-            pmod.launch("aw_p_obs", (1,), (256,), [chat.ptr, pw.ptr, ctypes.c_int(len(exprs_p)),
-                                                   ctypes.c_int(N), ring.ptr + (s * 8 + 4) * 8],
-                        current_stream())
-            # rho /= Re tr(rho) dX before the next step (applied by its opening pass)
-            self.plan.scale_from_stats(ring.ptr + s * 64, 8, 0, 0, self.dX, self._dscale)
-        for c0 in range(0, steps, CHUNK_STEPS):
-            self._run_steps_unmerged(ts[c0:c0 + CHUNK_STEPS], lambda q, c0=c0: observe(c0 + q))
+            self._ehr = dict(ts=empty((CH,), np.float64), Vt=empty((CH, N), np.float64),
+                             Kt=empty((CH, N), np.float64),
+                             fac=(p.new_factors(N, 2 * CH), p.new_factors(N, 2 * CH),
+                                  p.new_factors(N, CH), p.new_factors(N, CH)),
+                             xw=empty((CH, len(exprs_x), N), np.float64),
+                             pw=empty((CH, len(exprs_p), N), np.float64),
+                             csum=empty((N,), np.complex128), chat=empty((N,), np.complex128),
+                             stage=empty((CH, 8), np.float64), graphs={})
+            self._obs_module(exprs_x)
+            self._obs_module(exprs_p)
+            self._obs_module(["X"])
+            # the cyclic-diagonal scratch is allocated by its first call, never in a capture
+            self.plan.cyclic_diag_sums(self.rho, self._ehr["csum"])
+        bufs = self._ehr
+        if not self._scale_pending:
+            # every step applies the pending scale: start from 1 so that all chunks are alike
+            self._dscale.tensor[:, 0] = 1.0
+            self._dscale.tensor[:, 1] = 0.0
+            self._scale_pending = True
+        for c0 in range(0, steps, CH):
+            tsc = ts[c0:c0 + CH]
+            n = len(tsc)
+            self._chunk_factors(tsc, merged=False, bufs=bufs)
+            # the observable weights of all the chunk's times at once
+            self._weights_t(exprs_x, bufs["ts"], n, bufs["xw"])
+            self._weights_t(exprs_p, bufs["ts"], n, bufs["pw"])
+            graphs.run_chunk(bufs["graphs"], "ehrenfest", lambda n=n: self._ehrenfest_chunk(n), n == CH)
+            ring.tensor[c0:c0 + n].copy_(bufs["stage"].tensor[:n])
         self.t = ts[-1]
         self._flush_scale()
         r = ring.get()
@@ -377,7 +402,7 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
         mod = self._obs_cache.get(key)
         if mod is None:
             body = "".join("        out[%d * n + k] = (%s);\n" % (q, e) for q, e in enumerate(exprs))
-            src = (rtc.PRELUDE + self.cuda_consts
+            src = (rtc.PRELUDE + self.cuda_consts + "\n#define NOBS_W %d\n" % len(exprs)
                    + "\n__device__ double K(double P, double t)\n{\n    return (%s);\n}\n" % self.K
                    + "\n__device__ double V(double X, double t)\n{\n    return (%s);\n}\n" % self.V
                    + r"""
@@ -385,6 +410,16 @@ extern "C" __global__ void aw_weights(const double *Xg, const double *Pg, int n,
 {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const double X = Xg[k], P = Pg[k];
+%s    }
+}
+extern "C" __global__ void aw_weights_t(const double *Xg, const double *Pg, int n, const double *ts, int nt,
+                                        double *out_all)
+{
+    // the same weights for nt times: out_all[s][q][k]
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n * nt; idx += gridDim.x * blockDim.x) {
+        const int s = idx / n, k = idx - s * n;
+        const double X = Xg[k], P = Pg[k], t = ts[s];
+        double *out = out_all + (long long)s * NOBS_W * n;
 %s    }
 }
 extern "C" __global__ void aw_diag_obs(const %s *rho, const double *w, int nobs, int n, double *out)
@@ -432,7 +467,7 @@ extern "C" __global__ void aw_p_obs(const double2 *chat, const double *w, int no
         out[threadIdx.x] = s / n;
     }
 }
-""" % (body, self._state_t()))
+""" % (body, body, self._state_t()))
             mod = rtc.Module(src, "rho_vneumann_obs.hip")
             self._obs_cache[key] = mod
         return mod
@@ -448,9 +483,21 @@ extern "C" __global__ void aw_p_obs(const double2 *chat, const double *w, int no
                    current_stream())
 
     def _diag_obs(self, w, nobs, out_ptr):
+        self._diag_obs_ptr(w.ptr, nobs, out_ptr)
+
+    def _diag_obs_ptr(self, w_ptr, nobs, out_ptr):
         mod = self._obs_module(["X"])
-        mod.launch("aw_diag_obs", (1,), (256,), [self.rho.ptr, w.ptr, ctypes.c_int(nobs),
+        mod.launch("aw_diag_obs", (1,), (256,), [self.rho.ptr, w_ptr, ctypes.c_int(nobs),
                                                   ctypes.c_int(self.X_gridDIM), out_ptr],
+                   current_stream())
+
+    def _weights_t(self, exprs, ts_dev, nt, out):
+        """out[s][q][k] = expr_q(X_k, P_k, ts[s]) for s < nt (one launch per chunk)."""
+        N = self.X_gridDIM
+        mod = self._obs_module(exprs)
+        blocks, threads = rtc.launch_cfg(N * nt)
+        mod.launch("aw_weights_t", (blocks,), (threads,),
+                   [self._dev_kX.ptr, self._dev_kP.ptr, ctypes.c_int(N), ts_dev.ptr, ctypes.c_int(nt), out.ptr],
                    current_stream())
 
     def _copy_scalar(self, stats, comp, dst_ptr):

@@ -25,7 +25,7 @@ import numpy as np
 from . import _lib, rtc
 from .device import DeviceArray, current_stream, empty, require_gpu, to_device
 from .engine import FWD, INV, Plan, mem_info, rank1_split, sum_partials
-from . import wm_ops
+from . import graphs, wm_ops
 from .wm_ops import WMGrid
 
 CHUNK_STEPS = 256
@@ -118,6 +118,7 @@ class WignerMoyalCUDA1D:
             self.hamiltonian_average = []
         self._observable_kernels = {}
         self._obs_partials = {}
+        self._ehr = None          # persistent chunk buffers and captured graphs of the Ehrenfest loop
         self._prepare_masks()
         self.print_memory_info()
 
@@ -226,25 +227,36 @@ extern "C" __global__ void aw_eval_mask(int which, const double *xj, int nj, con
                    + "\n__device__ double K(double P, double t)\n{\n    return (%s);\n}\n" % self.K
                    + "\n__device__ double V(double X, double t)\n{\n    return (%s);\n}\n" % self.V
                    + r"""
-extern "C" __global__ void aw_obs(const %(st)s *W, long long n_item, double t, double *partials)
+__device__ __forceinline__ void aw_obs_body(const %(st)s *W, int n_item, double t, double *partials)
 {
     double acc[%(nobs)d];
     for (int c = 0; c < %(nobs)d; ++c) acc[c] = 0.;
     const %(st)s *Wb = W + (long long)blockIdx.y * n_item;
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n_item;
-         idx += (long long)gridDim.x * blockDim.x) {
-        const size_t i = idx / X_gridDIM, j = idx %% X_gridDIM;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n_item; idx += gridDim.x * blockDim.x) {
+        const int i = idx / X_gridDIM, j = idx - i * X_gridDIM;
         const double X = dX * (j - 0.5 * X_gridDIM);
         const double P = dP * (i - 0.5 * P_gridDIM);
         const double w = (double)Wb[idx].x;
 %(body)s    }
-""" % dict(st=self._state_t(), nobs=nobs, body=body) + rtc.REDUCE_TAIL % dict(nobs=nobs) + "}\n")
+""" % dict(st=self._state_t(), nobs=nobs, body=body) + rtc.REDUCE_TAIL % dict(nobs=nobs) + r"""}
+extern "C" __global__ void aw_obs(const %(st)s *W, int n_item, double t, double *partials)
+{
+    aw_obs_body(W, n_item, t, partials);
+}
+// the same with the time read from device memory (a captured step sequence replays with
+// new times written into the buffer)
+extern "C" __global__ void aw_obs_tp(const %(st)s *W, int n_item, const double *tp, double *partials)
+{
+    aw_obs_body(W, n_item, *tp, partials);
+}
+""" % dict(st=self._state_t()))
             mod = rtc.Module(src, "wigner_moyal_observables.hip")
             self._observable_kernels[key] = mod
         return mod
 
-    def _obs_launch(self, exprs, W, t, out):
-        """Partial sums of Re W * f_k(X, P, t) for each expression, summed into out[k]."""
+    def _obs_launch(self, exprs, W, t, out, t_ptr=None):
+        """Partial sums of Re W * f_k(X, P, t) for each expression, summed into out[k]
+        (t_ptr: a device double holding t instead of the value)."""
         mod = self._observable_kernel(exprs)
         n = self.X_gridDIM * self.P_gridDIM
         nblk = max(1, min(512, n // 2048))
@@ -253,8 +265,12 @@ extern "C" __global__ void aw_obs(const %(st)s *W, long long n_item, double t, d
         if partials is None:
             # reused by every launch of this shape (stream order keeps the uses apart)
             partials = self._obs_partials[key] = empty((self.batch, nblk, len(exprs)), np.float64)
-        mod.launch("aw_obs", (nblk, self.batch), (256,), [W.ptr, ctypes.c_longlong(n), ctypes.c_double(t),
-                                                          partials.ptr], current_stream())
+        if t_ptr is None:
+            mod.launch("aw_obs", (nblk, self.batch), (256,), [W.ptr, ctypes.c_int(n), ctypes.c_double(t),
+                                                              partials.ptr], current_stream())
+        else:
+            mod.launch("aw_obs_tp", (nblk, self.batch), (256,), [W.ptr, ctypes.c_int(n), t_ptr,
+                                                                 partials.ptr], current_stream())
         sum_partials(partials, nblk, len(exprs), self.batch, out)
 
     def _per_item(self, values):
@@ -411,51 +427,72 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             return m[1], m[2], None          # (v over lines-i, u over cols-j)
         return None, None, (m[1] if power == 1 else m[2])
 
-    def _build_chunk_factors(self, ts, merged=True):
+    def _build_chunk_factors(self, ts, merged=True, bufs=None):
         """Phase tables (FP64, hipRTC) and the per-pass complex factor vectors of one chunk.
         merged: the closing half-kick of a step and the opening one of the next share one
         column pass (n + 1 column vectors); otherwise each step has its own opening and
-        closing vectors (2 n column vectors, open at 2 s, close at 2 s + 1)."""
+        closing vectors (2 n column vectors, open at 2 s, close at 2 s + 1).  bufs: write
+        the times, tables and factors into these preallocated arrays (``_chunk_bufs``)."""
         n = len(ts)
         Nx, Np, dt = self.X_gridDIM, self.P_gridDIM, self.dt
-        ts_dev = to_device(ts)
-        Vj = empty((n, Nx), np.float64)
-        Vi = empty((n, Np), np.float64)
-        Kj = empty((n, Nx), np.float64)
-        Ki = empty((n, Np), np.float64)
+        if bufs is None:
+            ts_dev = to_device(ts)
+            Vj, Vi = empty((n, Nx), np.float64), empty((n, Np), np.float64)
+            Kj, Ki = empty((n, Nx), np.float64), empty((n, Np), np.float64)
+            dst = None
+        else:
+            ts_dev = bufs["ts"]
+            ts_dev[:n] = ts
+            Vj, Vi, Kj, Ki = bufs["Vj"], bufs["Vi"], bufs["Kj"], bufs["Ki"]
+            dst = bufs["fac"]
         self._eval_1d(0, self._dev_kXX, Nx, ts_dev, n, 0.5 * dt, Vj)
         self._eval_1d(0, self._dev_kXXp, Np, ts_dev, n, 0.5 * dt, Vi)
         self._eval_1d(1, self._dev_kPP, Nx, ts_dev, n, 0.5 * dt, Kj)
         self._eval_1d(1, self._dev_kPPp, Np, ts_dev, n, 0.5 * dt, Ki)
-        return self._factors_from_tables(n, Vj.ptr, Vi.ptr, Kj.ptr, Ki.ptr, dt, merged)
+        return self._factors_from_tables(n, Vj.ptr, Vi.ptr, Kj.ptr, Ki.ptr, dt, merged, dst=dst)
 
-    def _factors_from_tables(self, n, Vj, Vi, Kj, Ki, dt, merged=True, k_stride=None):
+    def _chunk_bufs(self, n):
+        """Persistent per-chunk inputs of the per-step loops (fixed addresses for capture)."""
+        Nx, Np, p = self.X_gridDIM, self.P_gridDIM, self.plan
+        return dict(ts=empty((n,), np.float64),
+                    Vj=empty((n, Nx), np.float64), Vi=empty((n, Np), np.float64),
+                    Kj=empty((n, Nx), np.float64), Ki=empty((n, Np), np.float64),
+                    fac=(p.new_factors(Nx, 2 * n), p.new_factors(Np, 2 * n),
+                         p.new_factors(Np, n), p.new_factors(Nx, n)))
+
+    def _factors_from_tables(self, n, Vj, Vi, Kj, Ki, dt, merged=True, k_stride=None, dst=None):
         """Factor vectors of n steps from FP64 phase tables (device pointers) V(XX_j),
         V(XX'_i) [n, Nx] / [n, Np] and K(PP_j), K(PP'_i) (k_stride 0: one row for all
-        steps).  A negative dt gives the conjugated factors: the adjoint step."""
+        steps).  A negative dt gives the conjugated factors: the adjoint step.  dst:
+        preallocated (colL, colE, rowL, rowE) with room for the vectors."""
         Nx, Np = self.X_gridDIM, self.P_gridDIM
         p = self.plan
         mxp, mx, _ = self._mask_args(0, 1)
         mpp, mp, _ = self._mask_args(1, 1)
         mxp_p = mxp.ptr if mxp is not None else None
         mx_p = mx.ptr if mx is not None else None
-        rowL = p.new_factors(Np, n)
-        rowE = p.new_factors(Nx, n)
+        if dst is not None:
+            colL, colE, rowL, rowE = dst
+        else:
+            rowL = p.new_factors(Np, n)
+            rowE = p.new_factors(Nx, n)
         # row passes: exp(-i dt [K(PP_j) - K(PP'_i)]) absp / Nx
         p.factors(rowL, Np, n, A=Ki, ca=dt, mag=mpp.ptr if mpp is not None else None,
                   scale=1.0 / Nx, a_stride=k_stride)
         p.factors(rowE, Nx, n, A=Kj, ca=-dt, mag=mp.ptr if mp is not None else None, a_stride=k_stride)
         if not merged:
-            colL = p.new_factors(Nx, 2 * n)
-            colE = p.new_factors(Np, 2 * n)
+            if dst is None:
+                colL = p.new_factors(Nx, 2 * n)
+                colE = p.new_factors(Np, 2 * n)
             for v, scale in ((0, 1.0), (1, 1.0 / Np)):
                 p.factors(colL, Nx, n, A=Vj, ca=-0.5 * dt, mag=mx_p, alt_sign=True,
                           out_offset=v, out_vstride=2)
                 p.factors(colE, Np, n, A=Vi, ca=0.5 * dt, mag=mxp_p, alt_sign=True, scale=scale,
                           out_offset=v, out_vstride=2)
             return colL, colE, rowL, rowE
-        colL = p.new_factors(Nx, n + 1)
-        colE = p.new_factors(Np, n + 1)
+        if dst is None:
+            colL = p.new_factors(Nx, n + 1)
+            colE = p.new_factors(Np, n + 1)
         # column passes: v = 0 start EV(t_0) (with S), 1..n-1 merged EV EV', n end EV (with S)
         p.factors(colL, Nx, 1, A=Vj, ca=-0.5 * dt, mag=mx_p, alt_sign=True)
         p.factors(colE, Np, 1, A=Vi, ca=0.5 * dt, mag=mxp_p, alt_sign=True)
@@ -518,26 +555,25 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         exprs = ["1.", "X", self.diff_K, "P", self.diff_V, self.hamiltonian]
         nobs = len(exprs)
         ring = empty((steps, self.batch, nobs), np.float64)
-        row_bytes = self.batch * nobs * 8
-        _, _, tab1 = self._mask_args(0, 1)
-        _, _, tabp = self._mask_args(1, 1)
-        p, rho = self.plan, self.rho
-        csz = np.dtype(self.state_dtype).itemsize
-        Nx, Np = self.X_gridDIM, self.P_gridDIM
-        for c0 in range(0, steps, CHUNK_STEPS):
-            tsc = ts[c0:c0 + CHUNK_STEPS]
-            colL, colE, rowL, rowE = self._build_chunk_factors(tsc, merged=False)
-            for k in range(len(tsc)):
-                s = c0 + k
-                # one step = three passes; its opening / closing vectors sit at 2 k and 2 k + 1
-                p.steps(rho, 1, colL.ptr + 2 * k * Nx * csz, colE.ptr + 2 * k * Np * csz,
-                        rowL.ptr + k * Np * csz, rowE.ptr + k * Nx * csz, mask_open_close=tab1,
-                        mask_row=tabp, dscale=self._dscale if self._scale_pending else None)
-                self._scale_pending = False
-                self.rho2wigner_()
-                self._obs_launch(exprs, self.wignerfunction, float(ts[s]), ring.ptr + s * row_bytes)
-                self.plan.scale_from_stats(ring.ptr + s * row_bytes, nobs, 0, 0, self.dXdP, self._dscale)
-                self._scale_pending = True
+        CH = CHUNK_STEPS
+        if self._ehr is None:
+            self._ehr = dict(self._chunk_bufs(CH), stage=empty((CH, self.batch, nobs), np.float64),
+                             graphs={})
+            self._observable_kernel(exprs)
+            self._obs_launch(exprs, self.wignerfunction, 0.0, self._ehr["stage"])   # allocates partials
+        bufs = self._ehr
+        if not self._scale_pending:
+            # every step applies the pending scale: start from 1 so that all chunks are alike
+            self._dscale.tensor[:, 0] = 1.0
+            self._dscale.tensor[:, 1] = 0.0
+            self._scale_pending = True
+        for c0 in range(0, steps, CH):
+            tsc = ts[c0:c0 + CH]
+            n = len(tsc)
+            self._build_chunk_factors(tsc, merged=False, bufs=bufs)
+            graphs.run_chunk(bufs["graphs"], tuple(exprs), lambda n=n: self._ehrenfest_chunk(exprs, n),
+                             n == CH)
+            ring.tensor[c0:c0 + n].copy_(bufs["stage"].tensor[:n])
         self.t = ts[-1]
         # final normalisation of rho and W by the last step's norm
         self.plan.scale(self.wignerfunction, self._dscale)
@@ -550,6 +586,30 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             vals = sgn * r[:, :, k] / s0
             lst.extend(vals[:, 0].tolist() if self.batch == 1 else list(vals))
         return self.wignerfunction
+
+    def _ehrenfest_chunk(self, exprs, n):
+        """n steps, each [EV, I0] [F1, EK, I1] [F0, EV] from the chunk buffers (applying the
+        pending normalisation), then the Wigner transform, the observables into the stage
+        ring and the next normalisation scale.  Only launches, from fixed addresses."""
+        bufs, p, rho = self._ehr, self.plan, self.rho
+        colL, colE, rowL, rowE = bufs["fac"]
+        _, _, tab1 = self._mask_args(0, 1)
+        _, _, tabp = self._mask_args(1, 1)
+        csz = np.dtype(self.state_dtype).itemsize
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        nobs = len(exprs)
+        row_bytes = self.batch * nobs * 8
+        stage, ts = bufs["stage"], bufs["ts"]
+        for k in range(n):
+            # one step = three passes; its opening / closing vectors sit at 2 k and 2 k + 1
+            p.steps(rho, 1, colL.ptr + 2 * k * Nx * csz, colE.ptr + 2 * k * Np * csz,
+                    rowL.ptr + k * Np * csz, rowE.ptr + k * Nx * csz, mask_open_close=tab1,
+                    mask_row=tabp, dscale=self._dscale)
+            self._scale_pending = False
+            self.rho2wigner_()
+            self._obs_launch(exprs, self.wignerfunction, 0.0, stage.ptr + k * row_bytes, t_ptr=ts.ptr + 8 * k)
+            p.scale_from_stats(stage.ptr + k * row_bytes, nobs, 0, 0, self.dXdP, self._dscale)
+            self._scale_pending = True
 
     def get_Ehrenfest(self, t):
         """wigner_moyal_cuda_1d.py:584-606 (normalises, then appends the five averages)."""
