@@ -7,10 +7,6 @@ namespace awig {
 
 constexpr int kLdsBudget = 160 * 1024;
 
-template <typename R, int N, bool STRIDED, int LPB, int TWM> constexpr int lds_bytes() {
-    return (LdsLayout<STRIDED, N, LPB>::SIZE + (TWM ? N : 0)) * int(sizeof(typename Cx<R>::T));
-}
-
 // Per (precision, line length, line kind):
 //  * E = 16 elements per thread (8 below 256), so a 1024-point line is 64 lanes, radix 16-16-4;
 //    FP32 columns of 512 and 1024 points use E = 32 (radix 32-16 / 32-32: one LDS exchange,
@@ -67,8 +63,7 @@ using PassKS = KCfg<R, N, SmallTune<R, N, STRIDED>::E, SmallTune<R, N, STRIDED>:
 
 template <typename C>
 int launch_cfg(const PassArgs &a, hipStream_t st) {
-    static_assert(lds_bytes<typename C::R, C::N, C::STRIDED, C::LPB, C::TWM>() <= kLdsBudget,
-                  "LDS budget");
+    static_assert(C::LDS_ELEMS * int(sizeof(typename Cx<typename C::R>::T)) <= kLdsBudget, "LDS budget");
     const long long blocks = (a.total_lines + C::LPB - 1) / C::LPB;
     if (blocks <= 0) return AW_OK;
     if (blocks > 0x7fffffffLL) return set_error(AW_ERR_INVALID_SIZE, "too many lines (%lld)", a.total_lines);

@@ -153,12 +153,12 @@ __device__ __forceinline__ void dft_reg(T2 *x) {
 //  columns (lanes interleave LPB adjacent lines): position-major [p][l] with one pad row per
 //  16 positions, so both the natural-order reads (u + T k) and the Stockham scatter writes
 //  (16 u + r) of a wave hit distinct banks.
-template <bool STRIDED, int N, int LPB> struct LdsLayout {
-    static constexpr int REGION = N + (N >> 5) + 1;
-    static constexpr int SIZE = STRIDED ? (N + (N >> 4)) * LPB : LPB * REGION;
+template <bool STRIDED, int N, int LPB, int PADC = 4, int PADR = 5> struct LdsLayout {
+    static constexpr int REGION = N + (N >> PADR) + 1;
+    static constexpr int SIZE = STRIDED ? (N + (N >> PADC)) * LPB : LPB * REGION;
     static __device__ __forceinline__ int at(int p, int l) {
-        if constexpr (STRIDED) return (p + (p >> 4)) * LPB + l;
-        else return l * REGION + p + (p >> 5);
+        if constexpr (STRIDED) return (p + (p >> PADC)) * LPB + l;
+        else return l * REGION + p + (p >> PADR);
     }
 };
 
@@ -228,7 +228,7 @@ __device__ __forceinline__ void fft_line(int dir, T2 *v, T2 *L, int l, int u, co
 // passes (GEN = 0) only multiply by line x element factor vectors (and an optional mask),
 // which keeps their code to two FFT bodies.
 template <typename R_, int N_, int E_, int LPB_, bool STRIDED_, int TWM_, int DA_ = 2, int DB_ = 2,
-          int GEN_ = 1, int MINW_ = 1, int PF_ = 0>
+          int GEN_ = 1, int MINW_ = 1, int PF_ = 0, int PADC_ = 4, int PADR_ = 5>
 struct KCfg {
     using R = R_;
     static constexpr int N = N_, E = E_, LPB = LPB_, TWM = TWM_, DA = DA_, DB = DB_, GEN = GEN_;
@@ -237,7 +237,7 @@ struct KCfg {
     static constexpr bool STRIDED = STRIDED_;
     static constexpr int T = N / E;
     static constexpr int THREADS = LPB * T;
-    using LAY = LdsLayout<STRIDED, N, LPB>;
+    using LAY = LdsLayout<STRIDED, N, LPB, PADC_, PADR_>;
     static constexpr int LDS_ELEMS = LAY::SIZE + (TWM != 0 ? N : 0);
 };
 
