@@ -81,6 +81,7 @@ class aw_pass(ctypes.Structure):
         ("estep", ctypes.c_double), ("ecenter", ctypes.c_double),
         ("lfftorder", ctypes.c_int), ("efftorder", ctypes.c_int),
         ("scale", ctypes.c_double), ("dscale", ctypes.c_void_p), ("out", ctypes.c_void_p),
+        ("ctab", ctypes.c_void_p), ("ctab_bstride", ctypes.c_longlong), ("chirp_out", ctypes.c_double),
     ]
 
 

@@ -162,7 +162,10 @@ int aw_pass_run(aw_plan *p, void *data, const aw_pass *ps, aw_stream stream) {
     a.efft = ps->efftorder;
     a.scale = ps->scale;
     a.dscale = ps->dscale;
-    a.has_factor = (ps->line_fac || ps->elem_fac || ps->mask || ps->chirp != 0.0 ||
+    a.ctab = ps->ctab;
+    a.c_bstride = ps->ctab_bstride;
+    a.chirp_out = ps->chirp_out;
+    a.has_factor = (ps->line_fac || ps->elem_fac || ps->mask || ps->ctab || ps->chirp != 0.0 ||
                     ps->scale != 1.0 || ps->dscale) ? 1 : 0;
     a.tw = strided ? p->tw_rows : p->tw_cols;
     return launch_pass(p->prec, n, strided, a, (hipStream_t)stream);

@@ -47,7 +47,8 @@ int launch_pass_f64(int n, bool strided, int kind, const PassArgs &a, hipStream_
 
 int launch_pass(int prec, int n, bool strided, const PassArgs &a, hipStream_t st) {
     int kind = 0;
-    const bool plain = a.chirp == 0.0 && !a.sign_in && !a.sign_out && a.out_mode == 0;
+    const bool plain = a.chirp == 0.0 && !a.sign_in && !a.sign_out && a.out_mode == 0 && !a.ctab &&
+                       a.chirp_out == 0.0;
     if (plain) {
         if (a.dir_a == 0 && a.dir_b == AW_INVERSE) kind = 1;
         else if (a.dir_a == AW_FORWARD && a.dir_b == AW_INVERSE) kind = 2;

@@ -31,6 +31,8 @@ struct PassArgs {
     double scale;
     const double *dscale;
     const void *tw;         // N roots exp(-2 pi i t / N), plan precision
+    const void *ctab; long long c_bstride;   // complex per-element factor table (general passes)
+    double chirp_out;                        // exp(i chirp_out lcoord ecoord) after the second FFT
 };
 
 struct FactorArgs {

@@ -350,9 +350,14 @@ void k_line_pass(PassArgs a) {
             for (int k = 0; k < E; ++k) v[k] = cmul(v[k], lf);
         }
         if (a.mask) {
-            const R *mk2 = reinterpret_cast<const R *>(a.mask) + item + off0;
+            const R *mk2 = reinterpret_cast<const R *>(a.mask) + b * a.m_bstride + off0;
 #pragma unroll
             for (int k = 0; k < E; ++k) v[k] = cscale(v[k], mk2[k * kstep]);
+        }
+        if (C::GEN && a.ctab) {
+            const T2 *ct = reinterpret_cast<const T2 *>(a.ctab) + b * a.c_bstride + off0;
+#pragma unroll
+            for (int k = 0; k < E; ++k) v[k] = cmul(v[k], ct[k * kstep]);
         }
         if (C::GEN && a.chirp != 0.0) {
             const int ql = a.lfft ? ((line + nl / 2) % nl) : line;
@@ -373,6 +378,20 @@ void k_line_pass(PassArgs a) {
         if (a.dir_b) fft_line<T2, N, E, LAY, C::TWM>(a.dir_b, v, L, l_loc, u, tw);
     } else if constexpr (C::DB != 0) {
         stockham<T2, N, E, 1, C::DB, LAY, C::TWM>(v, L, l_loc, u, tw);
+    }
+    if (C::GEN && a.chirp_out != 0.0) {
+        // a second chirp after the second FFT (the twiddles of a two-level transform)
+        const int ql = a.lfft ? ((line + nl / 2) % nl) : line;
+        const double lcoord = a.lstep * (double(ql) - a.lcenter);
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int e = u + T * k;
+            const int q = a.efft ? ((e + N / 2) % N) : e;
+            const double ph = a.chirp_out * (lcoord * (a.estep * (double(q) - a.ecenter)));
+            double sn, cs;
+            sincos(ph, &sn, &cs);
+            v[k] = cmul(v[k], mk<T2>(cs, sn));
+        }
     }
     if (C::GEN && a.sign_out) {
 #pragma unroll

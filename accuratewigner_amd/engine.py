@@ -55,7 +55,8 @@ class Plan:
     def run(self, data, axis, dir_a=0, dir_b=0, line_fac=None, elem_fac=None, mask=None,
             line_bstride=0, elem_bstride=0, mask_bstride=0, sign_in=0, sign_out=0,
             chirp=0.0, lstep=0.0, lcenter=0.0, estep=0.0, ecenter=0.0, lfft=0, efft=0,
-            scale=1.0, dscale=None, out=None, out_mode=0, stream=None):
+            scale=1.0, dscale=None, out=None, out_mode=0, stream=None, ctab=None, ctab_bstride=0,
+            chirp_out=0.0):
         ps = _lib.aw_pass()
         ps.axis, ps.dir_a, ps.dir_b = axis, dir_a, dir_b
         ps.sign_in, ps.sign_out, ps.out_mode = sign_in, sign_out, out_mode
@@ -67,6 +68,7 @@ class Plan:
         ps.scale = scale
         ps.dscale = _p(dscale)
         ps.out = _p(out)
+        ps.ctab, ps.ctab_bstride, ps.chirp_out = _p(ctab), ctab_bstride, chirp_out
         _lib.call("aw_pass_run", self.handle, ctypes.c_void_p(_p(data)), ctypes.byref(ps),
                   ctypes.c_void_p(stream if stream is not None else current_stream()))
 

@@ -9,6 +9,14 @@ t + dt/2, :805, :823) and P in FFT order (:803).  It runs as fused single-row pa
 with the per-step normalisation psi /= ||psi|| sqrt(dX) (:388) applied lazily on the device.
 get_wigner is implemented with the arrays and plans the reference forgot to create
 (:244, :255-256).
+
+Lines longer than 4096 points (e.g. the coherent strong-field driver's N = 32768,
+coherent_strong_field_physics_cuda_1d.py:31) use a two-level transform on the same memory
+viewed as [N1][N2] (n = n1 N2 + n2): column FFTs, twiddles exp(-2 pi i n2 k1 / N), row FFTs,
+which leaves frequency k = k1 + N1 k2 at position k1 N2 + k2.  The kinetic factors are
+evaluated on that permuted grid, so a step needs no transposes:
+[EV -> col FFT -> twiddle] ([row FFT -> EK -> row IFFT -> twiddle*] [col IFFT -> EV EV' ->
+col FFT -> twiddle])* ... [col IFFT -> EV].
 """
 import ctypes
 from types import FunctionType, MethodType
@@ -72,6 +80,19 @@ class SchrodingerWignerCUDA1D:
         # kernel coordinates from the #define'd steps (:803, :821)
         self._kX = rtc.kconst(self.dX) * (j - 0.5 * N)
         self._kP = rtc.kconst(self.dP) * (((j + N // 2) % N) - 0.5 * N)
+        # lines longer than 4096 points: two-level transform over [N1][N2]
+        self._two = None
+        self._kP_fac = self._kP
+        if N > 4096:
+            lg = int(np.log2(N))
+            N1 = 2 ** ((lg + 1) // 2)
+            N2 = N // N1
+            if N2 < 64 or N1 > 4096 or N2 > 4096:
+                raise ValueError("X_gridDIM = %d is outside the supported range" % N)
+            self._two = (N1, N2)
+            # momentum at the position where the two-level FFT leaves frequency k
+            m = np.arange(N)
+            self._kP_fac = self._kP[(m // N2) + N1 * (m % N2)]
 
         kwargs.update(dX=self.dX, dP=self.dP, P_max=self.P.max())
         self.cuda_consts = rtc.cuda_consts(kwargs, getattr(self, "consts", None), None)
@@ -84,12 +105,13 @@ class SchrodingerWignerCUDA1D:
         self.cuda_consts += getattr(self, "functions", "") or ""
 
         require_gpu()
-        self.plan = Plan(1, N, 1, self.precision)
+        self.plan = Plan(1, N, 1, self.precision) if self._two is None else Plan(self._two[0], self._two[1], 1,
+                                                                                  self.precision)
         self.state_dtype = self.plan.state_dtype
         self._wplan = None
         self._mod = rtc.Module(self._physics_source(), "schrodinger_physics.hip")
         self._dev_kX = to_device(self._kX)
-        self._dev_kP = to_device(self._kP)
+        self._dev_kP = to_device(self._kP_fac)
         self.wavefunction = DeviceArray((1, N), self.state_dtype)
         self._tmp = DeviceArray((1, N), self.state_dtype)
         self._dscale = empty((1, 2), np.float64)
@@ -193,14 +215,17 @@ extern "C" __global__ void aw_outer(const %s *psi, %s *rho, int n)
                    + "\n__device__ double K(double P, double t)\n{\n    return (%s);\n}\n" % self.K
                    + "\n__device__ double V(double X, double t)\n{\n    return (%s);\n}\n" % self.V
                    + r"""
-extern "C" __global__ void aw_weighted(const %(st)s *psi, int n, double t, double *out)
+extern "C" __global__ void aw_weighted(const %(st)s *psi, int n, double t, int perm_n2, double *out)
 {
-    // out[q] = sum_j |psi_j|^2 f_q(X_j, P_j, t) (weighted_func_cuda_code, :893-901); one block
+    // out[q] = sum_j |psi_j|^2 f_q(X_j, P_j, t) (weighted_func_cuda_code, :893-901); one block.
+    // perm_n2 > 0: psi is a two-level FFT output, position j holds frequency
+    // (j / perm_n2) + (n / perm_n2) (j %% perm_n2), which sets P
     double acc[%(nobs)d];
     for (int q = 0; q < %(nobs)d; ++q) acc[q] = 0.;
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const int kp = perm_n2 > 0 ? (j / perm_n2) + (n / perm_n2) * (j %% perm_n2) : j;
         const double X = dX * (j - 0.5 * X_gridDIM);
-        const double P = dP * ((j + X_gridDIM / 2) %% X_gridDIM - 0.5 * X_gridDIM);
+        const double P = dP * ((kp + X_gridDIM / 2) %% X_gridDIM - 0.5 * X_gridDIM);
         const double w = (double)psi[j].x * psi[j].x + (double)psi[j].y * psi[j].y;
 %(body)s    }
     __shared__ double red[4][%(nobs)d];
@@ -226,18 +251,53 @@ extern "C" __global__ void aw_inv_norm(const double *stats, double factor, doubl
             self._obs_cache[key] = mod
         return mod
 
-    def _weighted(self, exprs, out_ptr, psi=None, t=None):
+    def _weighted(self, exprs, out_ptr, psi=None, t=None, p_rep=False):
+        """p_rep: psi comes from ``_fft`` (permuted order for two-level lines)."""
         psi = self.wavefunction if psi is None else psi
         mod = self._obs_module(exprs)
+        perm = self._two[1] if (p_rep and self._two is not None) else 0
         mod.launch("aw_weighted", (1,), (256,), [psi.ptr, ctypes.c_int(self.X_gridDIM),
-                                                ctypes.c_double(float(self.t if t is None else t)), out_ptr],
+                                                ctypes.c_double(float(self.t if t is None else t)),
+                                                ctypes.c_int(perm), out_ptr],
                    current_stream())
+
+    def _fft(self, data, direction, out=None):
+        """Unnormalised DFT of the line (cuFFT convention).  Two-level lines: the forward
+        output is in the permuted order (frequency k1 + N1 k2 at position k1 N2 + k2) and
+        the inverse expects that order."""
+        if self._two is None:
+            self.plan.fft(data, 1, direction, out=out)
+            return
+        dst = data if out is None else out
+        if direction == FWD:
+            self._col(data, FWD, 0, twiddle=True, out=out)
+            self._row(dst, FWD, 0)
+        else:
+            self._row(data, INV, 0, twiddle=True, out=out)
+            self._col(dst, INV, 0)
+
+    def _col(self, data, a, b, ctab=None, twiddle=False, dscale=None, out=None):
+        """Two-level column pass; twiddle: exp(-2 pi i n2 k1 / N) after its FFTs."""
+        self.plan.run(data, 0, a, b, ctab=ctab, chirp_out=(-2.0 * np.pi / self.X_gridDIM) if twiddle else 0.0,
+                      lstep=1.0, lcenter=0.0, estep=1.0, ecenter=0.0, dscale=dscale, out=out)
+
+    def _row(self, data, a, b, ctab=None, twiddle=False, out=None):
+        """Two-level row pass; twiddle: exp(+2 pi i k1 n2 / N) after its FFTs."""
+        self.plan.run(data, 1, a, b, ctab=ctab, chirp_out=(2.0 * np.pi / self.X_gridDIM) if twiddle else 0.0,
+                      lstep=1.0, lcenter=0.0, estep=1.0, ecenter=0.0, out=out)
 
     def get_average(self, observable):
         """sum |psi_j|^2 f(X_j, P_j, t) with no normalisation (:392-410)."""
         self._flush_scale()
         out = empty((1,), np.float64)
         self._weighted([observable], out.ptr)
+        return float(out.get()[0])
+
+    def _p_average(self, observable):
+        """get_average of psi in the momentum representation left by ``_fft``."""
+        self._flush_scale()
+        out = empty((1,), np.float64)
+        self._weighted([observable], out.ptr, p_rep=True)
         return float(out.get()[0])
 
     # ------------------------------------------------------------------------------------
@@ -318,6 +378,9 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
         N = self.X_gridDIM
         csz = np.dtype(self.state_dtype).itemsize
         p, psi = self.plan, self.wavefunction
+        if self._two is not None:
+            self._run_two(ts, bloch, per_step, raw)
+            return
         if per_step is None:
             ev, ek = self._factors(ts, bloch, raw)
             p.run(psi, 1, 0, FWD, elem_fac=ev.ptr, dscale=self._dscale if self._scale_pending else None)
@@ -335,6 +398,36 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
             p.run(psi, 1, 0, INV, elem_fac=ek.ptr)
             p.run(psi, 1, 0, 0, elem_fac=ev.ptr + N * csz)
             per_step(s)
+
+    def _run_two(self, ts, bloch, per_step, raw):
+        """_run for two-level lines: EV and EK are full-line tables applied inside the passes."""
+        n = len(ts)
+        N = self.X_gridDIM
+        csz = np.dtype(self.state_dtype).itemsize
+        psi = self.wavefunction
+        if per_step is None:
+            ev, ek = self._factors(ts, bloch, raw)
+            self._col(psi, 0, FWD, ctab=ev.ptr, twiddle=True,
+                      dscale=self._dscale if self._scale_pending else None)
+            self._scale_pending = False
+            for s in range(n):
+                self._row(psi, FWD, INV, ctab=ek.ptr + s * N * csz, twiddle=True)
+                last = s == n - 1
+                self._col(psi, INV, 0 if last else FWD, ctab=ev.ptr + (s + 1) * N * csz, twiddle=not last)
+            return
+        for s in range(n):
+            ev, ek = self._factors(ts[s:s + 1], bloch)
+            self._two_step(ev, ek)
+            per_step(s)
+
+    def _two_step(self, ev, ek):
+        """One unmerged two-level step (three passes) from factor tables ev[0..1], ek[0]."""
+        N, psi = self.X_gridDIM, self.wavefunction
+        csz = np.dtype(self.state_dtype).itemsize
+        self._col(psi, 0, FWD, ctab=ev.ptr, twiddle=True, dscale=self._dscale if self._scale_pending else None)
+        self._scale_pending = False
+        self._row(psi, FWD, INV, ctab=ek.ptr, twiddle=True)
+        self._col(psi, INV, 0, ctab=ev.ptr + N * csz)
 
     def single_step_propagation(self):
         """:356-369 (unnormalised)."""
@@ -364,8 +457,8 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
             k = c0 + s
             t = float(ts[k])
             self._weighted(xs, ring.ptr + k * 64, t=t)
-            self.plan.fft(self.wavefunction, 1, FWD, out=self._tmp)
-            self._weighted(ps, ring.ptr + k * 64 + 32, psi=self._tmp, t=t)
+            self._fft(self.wavefunction, FWD, out=self._tmp)
+            self._weighted(ps, ring.ptr + k * 64 + 32, psi=self._tmp, t=t, p_rep=True)
             self._norm_scale(np.sqrt(self.dX))
             self._scale_pending = True
 
@@ -390,21 +483,21 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
         self.X_average.append(self.get_average("X"))
         self.P_average_RHS.append(-self.get_average(self.diff_V))
         self.hamiltonian_average.append(self.get_average(self.V))
-        self.plan.fft(self.wavefunction, 1, FWD)
+        self._fft(self.wavefunction, FWD)
         self._normalize(1.0)
-        self.X_average_RHS.append(self.get_average(self.diff_K))
-        self.P_average.append(self.get_average("P"))
-        self.hamiltonian_average[-1] += self.get_average(self.K)
-        self.plan.fft(self.wavefunction, 1, INV)
+        self.X_average_RHS.append(self._p_average(self.diff_K))
+        self.P_average.append(self._p_average("P"))
+        self.hamiltonian_average[-1] += self._p_average(self.K)
+        self._fft(self.wavefunction, INV)
 
     def get_energy(self):
         """:461-482 (leaves psi with 2-norm sqrt(N), like the reference)."""
         self._normalize(1.0)
         av_V = self.get_average(self.V)
-        self.plan.fft(self.wavefunction, 1, FWD)
+        self._fft(self.wavefunction, FWD)
         self._normalize(1.0)
-        av_K = self.get_average(self.K)
-        self.plan.fft(self.wavefunction, 1, INV)
+        av_K = self._p_average(self.K)
+        self._fft(self.wavefunction, INV)
         return av_V + av_K
 
     def set_ground_state(self, nsteps=10000):
@@ -447,9 +540,12 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
                 self.set_wavefunction("atan(X)")
             even = not even
             for _ in range(nsteps):
-                p.run(psi, 1, 0, FWD, elem_fac=ev.ptr)
-                p.run(psi, 1, 0, INV, elem_fac=ek.ptr)
-                p.run(psi, 1, 0, 0, elem_fac=ev.ptr + N * csz)
+                if self._two is None:
+                    p.run(psi, 1, 0, FWD, elem_fac=ev.ptr)
+                    p.run(psi, 1, 0, INV, elem_fac=ek.ptr)
+                    p.run(psi, 1, 0, 0, elem_fac=ev.ptr + N * csz)
+                else:
+                    self._two_step(ev, ek)
                 self.projectout_stationary_states(psi)
             self._phis.tensor[n].copy_(psi.tensor[0])
             self.stationary_states.append(psi.copy())
@@ -463,6 +559,8 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
         [F0, SY, I0]; [F1, SX, I1]; (-1)^i, I0, (-1)^i; W /= Re sum W dX dP."""
         self._flush_scale()
         N = self.X_gridDIM
+        if self._two is not None:
+            raise NotImplementedError("get_wigner needs an N x N grid; X_gridDIM = %d is above 4096" % N)
         if self._wplan is None:
             self._wplan = Plan(N, N, 1, self.precision)
             self.wignerfunction = DeviceArray((N, N), self.state_dtype)

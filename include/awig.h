@@ -71,13 +71,15 @@ int aw_fft_axis(aw_plan *plan, void *data, void *out, int axis, int dir, aw_stre
  *          -> store (or store the diagonal element only)
  * with factor(line l, element e) =
  *     scale * dscale[b] * line_fac[b][l] * elem_fac[b][e] * mask[b][row, col]
- *     * exp(i * chirp * lcoord(l) * ecoord(e)),
+ *     * ctab[b][row, col] * exp(i * chirp * lcoord(l) * ecoord(e)),
+ * and, after the second FFT, a multiply by exp(i * chirp_out * lcoord(l) * ecoord(e)) (the
+ * inter-level twiddles of a line longer than 4096 points split over rows and columns).
  * lcoord(l) = lstep * (lpos(l) - lcenter), ecoord likewise, where lpos is l itself or its
  * FFT-ordered index ((l + n/2) % n - n/2 + lcenter) when lfftorder is set.
  * It replaces the reference's expV / expK / bloch_expV / bloch_expK / phase_shearX /
  * phase_shearY / sign_flip / observable kernels together with the cuFFT calls around
  * them, e.g. the sequence of single_step_propagation (wigner_moyal_cuda_1d.py:543-561).
- * Vectors are complex in the plan's precision; mask is real in the plan's precision.
+ * Vectors and ctab are complex in the plan's precision; mask is real in the plan's precision.
  * NULL pointers mean "all ones".  out = NULL or data means in place.
  */
 typedef struct aw_pass {
@@ -93,6 +95,8 @@ typedef struct aw_pass {
     double scale;
     const double *dscale; /* complex (re, im) per batch item, or NULL */
     void *out;
+    const void *ctab; long long ctab_bstride;         /* complex [rows][cols] factor table */
+    double chirp_out;
 } aw_pass;
 
 int aw_pass_run(aw_plan *plan, void *data, const aw_pass *pass, aw_stream stream);
