@@ -136,17 +136,12 @@ def test_wb_harmonic_gibbs_exact():
 
 
 def exact_thermal_energy(V, beta, L=10., N=256):
-    """<H>(beta) by exact diagonalisation on a position grid with the spectral kinetic
-    operator (the MUB Hamiltonian pattern of band_structure_1d.py:64-80)."""
-    dx = 2 * L / N
-    x = -L + dx * np.arange(N)
-    p = 2 * np.pi * np.fft.fftfreq(N, dx)
-    F = np.fft.fft(np.eye(N), axis=0)
-    T = (np.fft.ifft(0.5 * p[:, None] ** 2 * F, axis=0)).real
-    H = T + np.diag(V(x, 0.))
-    E = np.linalg.eigvalsh(0.5 * (H + H.T))
-    w = np.exp(-beta * (E - E[0]))
-    return float(np.sum(E * w) / np.sum(w)), float(E[0])
+    """<H>(beta) and E0 by exact diagonalisation of the MUB Hamiltonian
+    (oracle/band_structure.py, band_structure_1d.py:64-92)."""
+    from oracle.band_structure import MUBBandStructure
+    b = MUBBandStructure(N, L, lambda x: V(x, 0.), lambda p: 0.5 * p * p)
+    E = np.linalg.eigvalsh(b.get_hamiltonian(0.0))
+    return b.thermal_energy(beta), float(E[0])
 
 
 def test_wb_anharmonic_energy_vs_diagonalisation():
