@@ -69,3 +69,36 @@ def test_rtc_compiles_reference_style_expressions():
     assert ok, log
     bad_ok, bad_log = rtc.check_source(src + "\n__device__ double broken( { }\n")
     assert not bad_ok and "error" in bad_log.lower()
+
+
+def test_entry_points_reject_null_and_bad_arguments_before_touching_the_gpu():
+    """Argument checks run before any HIP call, so they hold on a machine without a GPU."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libawig.so not built")
+    import ctypes
+    lib = _lib.load()
+    vp = ctypes.c_void_p
+    steps = _lib.aw_steps()
+    steps.nsteps = 1
+    ps = _lib.aw_pass()
+    cases = [
+        (_lib.awigInvalidPlan, lambda: lib.aw_step_sequence(None, vp(1), ctypes.byref(steps), None)),
+        (_lib.awigInvalidPlan, lambda: lib.aw_pass_run(None, vp(1), ctypes.byref(ps), None)),
+        (_lib.awigInvalidPlan, lambda: lib.aw_pair_dot(None, vp(1), vp(1), vp(1), vp(1), vp(1), None)),
+        (_lib.awigInvalidPlan, lambda: lib.aw_cyclic_diag_sums(None, vp(1), vp(1), None)),
+        (_lib.awigInvalidPlan, lambda: lib.aw_reduce_stats(None, vp(1), vp(1), None)),
+        (_lib.awigInvalidPlan, lambda: lib.aw_make_factors(None, vp(1), 0, 8, 1, 0, None, 8, 0., None, 8, 0.,
+                                                            0., None, 0, 1, 1., 0, None)),
+        (_lib.awigInvalidValue, lambda: lib.aw_sum_partials(None, 1, 1, 1, vp(1), None)),
+        (_lib.awigInvalidValue, lambda: lib.aw_scale_from_stats(None, 8, 0, 0, 1., vp(1), 1, None)),
+        (_lib.awigInvalidValue, lambda: lib.aw_project_out(0, vp(1), None, 1, 128, None)),
+        (_lib.awigInvalidValue, lambda: lib.aw_project_out(0, vp(1), vp(1), 5000, 128, None)),
+        (_lib.awigInvalidValue, lambda: lib.aw_reduce_vec(0, None, 8, 1, vp(1), None)),
+        (_lib.awigInvalidType, lambda: lib.aw_reduce_vec(9, vp(1), 8, 1, vp(1), None)),
+        (_lib.awigInvalidSize, lambda: lib.aw_plan_create(ctypes.byref(vp()), 64, 8192, 1, 0)),
+        (_lib.awigInvalidSize, lambda: lib.aw_plan_create(ctypes.byref(vp()), 32, 64, 1, 0)),
+    ]
+    for exc, call in cases:
+        with pytest.raises(exc):
+            _lib.check(call())
+        assert _lib.load().aw_last_error()          # a message is left for the caller
