@@ -16,7 +16,7 @@ the reference plots; ``ehrenfest/...``).
 """
 import numpy as np
 
-from .strong_field import NpzWriter, hhg_spectrum  # noqa: F401  (re-exported for callers)
+from .strong_field import NpzWriter, hhg_spectrum  # noqa: F401  (the HHG spectrum of <P>, :334-358)
 
 
 def coulomb_sys_params(X_gridDIM=16 * 2048, X_amplitude=2000., dt=0.01):

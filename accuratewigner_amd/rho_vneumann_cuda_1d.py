@@ -15,7 +15,7 @@ import numpy as np
 
 from . import graphs, rtc
 from .device import DeviceArray, current_stream, empty, require_gpu, to_device
-from .engine import FWD, INV, Plan, mem_info, reduce_vec, sum_partials
+from .engine import FWD, INV, Plan, mem_info
 
 SQRT2 = np.sqrt(2.0)
 CHUNK_STEPS = 256

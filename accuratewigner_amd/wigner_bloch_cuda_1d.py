@@ -10,7 +10,7 @@ step: [EVb, I0] [F1, EKb, I1] [F0, EVb].
 """
 import numpy as np
 
-from .device import current_stream, empty, to_device
+from .device import empty, to_device
 from .engine import FWD, INV
 from .wigner_moyal_cuda_1d import WignerMoyalCUDA1D
 

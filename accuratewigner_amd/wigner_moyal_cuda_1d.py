@@ -22,7 +22,7 @@ from types import FunctionType, MethodType
 
 import numpy as np
 
-from . import _lib, rtc
+from . import rtc
 from .device import DeviceArray, current_stream, empty, require_gpu, to_device
 from .engine import FWD, INV, Plan, mem_info, rank1_split, sum_partials
 from . import graphs, wm_ops
