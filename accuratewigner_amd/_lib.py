@@ -109,6 +109,7 @@ _SIGNATURES = {
     "aw_fft_axis": [_vp, _vp, _vp, _i, _i, _vp],
     "aw_pass_run": [_vp, _vp, ctypes.POINTER(aw_pass), _vp],
     "aw_step_sequence": [_vp, _vp, ctypes.POINTER(aw_steps), _vp],
+    "aw_step_group": [_vp, _i, _i, ctypes.POINTER(_i)],
     "aw_make_factors": [_vp, _vp, _ll, _i, _i, _i, _vp, _ll, _d, _vp, _ll, _d, _d, _vp, _ll, _i,
                         _d, _i, _vp],
     "aw_reduce_stats": [_vp, _vp, _vp, _vp],

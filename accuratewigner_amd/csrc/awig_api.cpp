@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -120,8 +121,9 @@ int aw_plan_destroy(aw_plan *p) {
     return AW_OK;
 }
 
-int aw_pass_run(aw_plan *p, void *data, const aw_pass *ps, aw_stream stream) {
-    if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
+// One pass over the first `nbatch` items at `data` (the plan's batch for aw_pass_run; a group
+// of items for the step sequences).
+static int pass_run_n(aw_plan *p, void *data, const aw_pass *ps, int nbatch, aw_stream stream) {
     if (!data || !ps) return set_error(AW_ERR_INVALID_VALUE, "NULL data or pass");
     if (ps->axis != 0 && ps->axis != 1) return set_error(AW_ERR_INVALID_VALUE, "axis must be 0 or 1");
     if (ps->axis == 0 && p->rows == 1)
@@ -140,7 +142,7 @@ int aw_pass_run(aw_plan *p, void *data, const aw_pass *ps, aw_stream stream) {
     a.nlines = strided ? p->cols : p->rows;
     a.line_stride = strided ? 1 : p->cols;
     a.elem_stride = strided ? p->cols : 1;
-    a.total_lines = (long long)a.nlines * p->batch;
+    a.total_lines = (long long)a.nlines * nbatch;
     a.dir_a = ps->dir_a;
     a.dir_b = ps->dir_b;
     a.sign_in = ps->sign_in;
@@ -171,6 +173,44 @@ int aw_pass_run(aw_plan *p, void *data, const aw_pass *ps, aw_stream stream) {
     return launch_pass(p->prec, n, strided, a, (hipStream_t)stream);
 }
 
+int aw_pass_run(aw_plan *p, void *data, const aw_pass *ps, aw_stream stream) {
+    if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
+    return pass_run_n(p, data, ps, p->batch, stream);
+}
+
+// Bytes of state a step sequence keeps in flight per group of items.  Every pass of a
+// sequence reads and writes the whole state; once the batch outgrows the 256 MB MALL (the
+// memory-side Infinity Cache) each pass streams it from HBM.  Stepping the batch in groups
+// of at most this many bytes through all the passes of the sequence keeps a group resident
+// in the MALL between passes.  Items are independent and every line computes the same
+// arithmetic in any grouping, so results are bit-identical to one launch per pass.
+// Measured at 1024^2 x 96 complex64, 129 passes: 24-item groups (192 MiB) 1.45e11
+// gp-steps/s against 1.28e11 for the whole batch (row pass 4.64 -> 4.98 TB/s of
+// algorithmic traffic, column pass 3.54 -> 4.26 TB/s); 8-item groups are slower again
+// (too few lines per launch).
+constexpr long long kGroupBytes = 192ll << 20;
+constexpr long long kGroupMinLines = 8192;   // keep >= 1024 blocks per launch
+
+// items per group: the whole batch unless it outgrows kGroupBytes (only worth it for several
+// steps, and only while a group still fills the GPU); equal groups otherwise
+static int step_group(const aw_plan *p, int nsteps, int active) {
+    const long long fit = kGroupBytes / ((long long)p->rows * p->cols * p->csize);
+    if (nsteps < 2 || fit < 1 || active <= fit || fit * std::min(p->rows, p->cols) < kGroupMinLines)
+        return active;
+    const long long ngroups = (active + fit - 1) / fit;
+    return int((active + ngroups - 1) / ngroups);
+}
+
+int aw_step_group(const aw_plan *p, int nsteps, int batch, int *group) {
+    if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
+    if (!group) return set_error(AW_ERR_INVALID_VALUE, "NULL group");
+    if (nsteps < 1) return set_error(AW_ERR_INVALID_VALUE, "nsteps must be >= 1");
+    if (batch < 0 || batch > p->batch)
+        return set_error(AW_ERR_INVALID_VALUE, "batch %d outside 0..%d", batch, p->batch);
+    *group = step_group(p, nsteps, batch > 0 ? batch : p->batch);
+    return AW_OK;
+}
+
 int aw_step_sequence(aw_plan *p, void *data, const aw_steps *st, aw_stream stream) {
     if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
     if (!data || !st) return set_error(AW_ERR_INVALID_VALUE, "NULL data or steps");
@@ -183,37 +223,50 @@ int aw_step_sequence(aw_plan *p, void *data, const aw_steps *st, aw_stream strea
     const char *rl = static_cast<const char *>(st->row_line), *re = static_cast<const char *>(st->row_elem);
     if (!cl || !ce || !rl || !re) return set_error(AW_ERR_INVALID_VALUE, "NULL factor table");
     const int n = st->nsteps;
-    const int saved_batch = p->batch;
-    if (st->batch > 0) p->batch = st->batch;   // run the active prefix only
+    const int active = st->batch > 0 ? st->batch : p->batch;   // a slot pool runs its active prefix
+    const long long item_elems = (long long)p->rows * p->cols;
+    const int group = step_group(p, n, active);
     // column vector v of the sequence and row vector s (constant tables reuse 0/1/2 and 0)
     auto cvec = [&](int v) { return st->const_steps ? (v == 0 ? 0 : (v == n ? 2 : 1)) : v; };
     auto rvec = [&](int s) { return st->const_steps ? 0 : s; };
-    aw_pass ps{};
-    ps.scale = 1.0;
-    ps.mask_bstride = st->mask_bstride;
-    // opening column pass: factor c_0 then inverse FFT along axis 0
-    ps.axis = 0; ps.dir_a = 0; ps.dir_b = AW_INVERSE;
-    ps.line_fac = cl; ps.line_fac_bstride = st->col_line_bstride;
-    ps.elem_fac = ce; ps.elem_fac_bstride = st->col_elem_bstride;
-    ps.mask = st->mask_open_close;
-    ps.dscale = st->dscale;
-    int rc = aw_pass_run(p, data, &ps, stream);
-    ps.dscale = nullptr;
-    for (int s = 0; rc == AW_OK && s < n; ++s) {
-        ps.axis = 1; ps.dir_a = AW_FORWARD; ps.dir_b = AW_INVERSE;
-        ps.line_fac = rl + rvec(s) * p->rows * esz; ps.line_fac_bstride = st->row_line_bstride;
-        ps.elem_fac = re + rvec(s) * p->cols * esz; ps.elem_fac_bstride = st->row_elem_bstride;
-        ps.mask = st->mask_row;
-        rc = aw_pass_run(p, data, &ps, stream);
-        if (rc != AW_OK) break;
-        const bool last = s == n - 1;
-        ps.axis = 0; ps.dir_a = AW_FORWARD; ps.dir_b = last ? 0 : AW_INVERSE;
-        ps.line_fac = cl + cvec(s + 1) * p->cols * esz; ps.line_fac_bstride = st->col_line_bstride;
-        ps.elem_fac = ce + cvec(s + 1) * p->rows * esz; ps.elem_fac_bstride = st->col_elem_bstride;
-        ps.mask = last ? st->mask_open_close : st->mask_mid;
-        rc = aw_pass_run(p, data, &ps, stream);
+    const long long rsz = esz / 2;   // masks are real
+    int rc = AW_OK;
+    for (int b0 = 0; rc == AW_OK && b0 < active; b0 += group) {
+        const int nb = std::min(group, active - b0);
+        // per-item tables advance by their batch strides (0: shared by all items)
+        char *d = static_cast<char *>(data) + b0 * item_elems * esz;
+        const char *gcl = cl + b0 * st->col_line_bstride * esz, *gce = ce + b0 * st->col_elem_bstride * esz;
+        const char *grl = rl + b0 * st->row_line_bstride * esz, *gre = re + b0 * st->row_elem_bstride * esz;
+        auto moff = [&](const void *m) -> const void * {
+            return m ? static_cast<const char *>(m) + b0 * st->mask_bstride * rsz : nullptr;
+        };
+        const void *m_oc = moff(st->mask_open_close), *m_mid = moff(st->mask_mid), *m_row = moff(st->mask_row);
+        aw_pass ps{};
+        ps.scale = 1.0;
+        ps.mask_bstride = st->mask_bstride;
+        // opening column pass: factor c_0 then inverse FFT along axis 0
+        ps.axis = 0; ps.dir_a = 0; ps.dir_b = AW_INVERSE;
+        ps.line_fac = gcl; ps.line_fac_bstride = st->col_line_bstride;
+        ps.elem_fac = gce; ps.elem_fac_bstride = st->col_elem_bstride;
+        ps.mask = m_oc;
+        ps.dscale = st->dscale ? st->dscale + 2 * b0 : nullptr;
+        rc = pass_run_n(p, d, &ps, nb, stream);
+        ps.dscale = nullptr;
+        for (int s = 0; rc == AW_OK && s < n; ++s) {
+            ps.axis = 1; ps.dir_a = AW_FORWARD; ps.dir_b = AW_INVERSE;
+            ps.line_fac = grl + rvec(s) * p->rows * esz; ps.line_fac_bstride = st->row_line_bstride;
+            ps.elem_fac = gre + rvec(s) * p->cols * esz; ps.elem_fac_bstride = st->row_elem_bstride;
+            ps.mask = m_row;
+            rc = pass_run_n(p, d, &ps, nb, stream);
+            if (rc != AW_OK) break;
+            const bool last = s == n - 1;
+            ps.axis = 0; ps.dir_a = AW_FORWARD; ps.dir_b = last ? 0 : AW_INVERSE;
+            ps.line_fac = gcl + cvec(s + 1) * p->cols * esz; ps.line_fac_bstride = st->col_line_bstride;
+            ps.elem_fac = gce + cvec(s + 1) * p->rows * esz; ps.elem_fac_bstride = st->col_elem_bstride;
+            ps.mask = last ? m_oc : m_mid;
+            rc = pass_run_n(p, d, &ps, nb, stream);
+        }
     }
-    p->batch = saved_batch;
     return rc;
 }
 

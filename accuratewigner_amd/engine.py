@@ -29,6 +29,21 @@ def _p(x):
     return x.data_ptr()
 
 
+_DEFERRED_PLANS = []
+
+
+def _capturing():
+    """True while the current stream is being captured into a graph."""
+    import torch
+    return torch.cuda.is_available() and torch.cuda.is_current_stream_capturing()
+
+
+def destroy_deferred_plans():
+    """Destroy the plans that were dropped while a graph capture was open."""
+    while _DEFERRED_PLANS and not _capturing():
+        _lib.load().aw_plan_destroy(_DEFERRED_PLANS.pop())
+
+
 class Plan:
     """Batched [rows, cols] engine (replaces the cuFFT plans Plan_Z2Z_2D_Axis0/1,
     cufft.py:274-345, and the PyCUDA elementwise kernels around them)."""
@@ -45,9 +60,13 @@ class Plan:
 
     def __del__(self):
         try:
-            if getattr(self, "handle", None):
-                _lib.load().aw_plan_destroy(self.handle)
-                self.handle = None
+            h = getattr(self, "handle", None)
+            self.handle = None
+            if h:
+                if _capturing():
+                    _DEFERRED_PLANS.append(h)      # no hipFree inside a graph capture
+                else:
+                    _lib.load().aw_plan_destroy(h)
         except Exception:
             pass
 
@@ -90,6 +109,13 @@ class Plan:
         st.dscale = _p(dscale)
         _lib.call("aw_step_sequence", self.handle, ctypes.c_void_p(_p(data)), ctypes.byref(st),
                   ctypes.c_void_p(stream if stream is not None else current_stream()))
+
+    def step_group(self, nsteps, batch=0):
+        """Items per group that ``steps`` uses for nsteps steps over ``batch`` items (0: all):
+        batches over 192 MiB run in groups that stay in the MALL between passes."""
+        g = ctypes.c_int()
+        _lib.call("aw_step_group", self.handle, int(nsteps), int(batch), ctypes.byref(g))
+        return g.value
 
     def fft(self, data, axis, direction, out=None):
         _lib.call("aw_fft_axis", self.handle, ctypes.c_void_p(_p(data)),

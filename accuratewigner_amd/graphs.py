@@ -8,8 +8,14 @@ stream, which every libawig launch uses) and replayed for every later chunk afte
 inputs are rewritten.  Captured sequences contain kernel launches only: no allocation,
 memset or host synchronisation.
 
+The Python cyclic collector is run before a capture and held off during it: a collection
+inside the capture could finalise an unrelated dropped object whose finaliser frees device
+memory (hipFree), which a capture in global mode does not allow (the capture is then
+invalidated).  Plans dropped while a capture is open are destroyed after it (engine.Plan).
+
 Set AWIG_GRAPHS=0 to run every chunk eagerly (same kernels, same order, same results).
 """
+import gc
 import os
 
 
@@ -22,9 +28,18 @@ class Captured:
 
     def __init__(self, fn):
         import torch
+        from . import engine
         self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
-            fn()
+        gc.collect()
+        was_enabled = gc.isenabled()
+        gc.disable()
+        try:
+            with torch.cuda.graph(self.graph):
+                fn()
+        finally:
+            if was_enabled:
+                gc.enable()
+            engine.destroy_deferred_plans()
 
     def replay(self):
         self.graph.replay()

@@ -9,11 +9,12 @@ passes [F0, EV EV', I0] [F1, EK, I1]; the trace renormalisation after every step
 positive real scalar, applied lazily on the device.
 """
 import ctypes
-from types import FunctionType, MethodType
+from types import FunctionType
 
 import numpy as np
 
 from . import graphs, rtc
+from ._params import KeywordParams
 from .device import DeviceArray, current_stream, empty, require_gpu, to_device
 from .engine import FWD, INV, Plan, mem_info
 
@@ -21,7 +22,7 @@ SQRT2 = np.sqrt(2.0)
 CHUNK_STEPS = 256
 
 
-class RhoVNeumannCUDA1D:
+class RhoVNeumannCUDA1D(KeywordParams):
     """Second-order split-operator propagator of the von Neumann equation for rho(x, x', t)
     with H = K(p, t) + V(x, t) (reference: rho_vneumann_cuda_1d.py:15-21), on MI355X."""
 
@@ -32,11 +33,7 @@ class RhoVNeumannCUDA1D:
         kwargs = dict(kwargs)
         self.precision = kwargs.pop("precision", "fp64")
         self.batch = 1
-        for name, value in kwargs.items():
-            if isinstance(value, FunctionType):
-                setattr(self, name, MethodType(value, self))
-            else:
-                setattr(self, name, value)
+        self._set_params(kwargs)
         if not hasattr(self, "X_gridDIM"):
             raise AttributeError("Coordinate grid size (X_gridDIM) was not specified")
         assert 2 ** int(np.log2(self.X_gridDIM)) == self.X_gridDIM, \

@@ -19,11 +19,12 @@ evaluated on that permuted grid, so a step needs no transposes:
 col FFT -> twiddle])* ... [col IFFT -> EV].
 """
 import ctypes
-from types import FunctionType, MethodType
+from types import FunctionType
 
 import numpy as np
 
 from . import _lib, rtc
+from ._params import KeywordParams
 from .device import DeviceArray, current_stream, empty, require_gpu, to_device
 from .engine import FWD, INV, Plan, mem_info, reduce_vec
 
@@ -31,7 +32,7 @@ SQRT2 = np.sqrt(2.0)
 CHUNK_STEPS = 256
 
 
-class SchrodingerWignerCUDA1D:
+class SchrodingerWignerCUDA1D(KeywordParams):
     """Second-order split-operator propagator of the Schrodinger equation with
     H = K(p, t) + V(x, t) (reference: schrodinger_wigner_cuda_1d.py:16-25), on MI355X."""
 
@@ -39,11 +40,7 @@ class SchrodingerWignerCUDA1D:
         """Keywords as schrodinger_wigner_cuda_1d.py:26-49, plus ``precision``."""
         kwargs = dict(kwargs)
         self.precision = kwargs.pop("precision", "fp64")
-        for name, value in kwargs.items():
-            if isinstance(value, FunctionType):
-                setattr(self, name, MethodType(value, self))
-            else:
-                setattr(self, name, value)
+        self._set_params(kwargs)
         if not hasattr(self, "X_gridDIM"):
             raise AttributeError("Coordinate grid size (X_gridDIM) was not specified")
         assert 2 ** int(np.log2(self.X_gridDIM)) == self.X_gridDIM, \

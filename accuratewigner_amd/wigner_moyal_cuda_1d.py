@@ -18,11 +18,12 @@ arguments and FP64 reductions); ``batch`` = B > 1 propagates B independent state
 same Hamiltonian in every launch (arrays become [B, Np, Nx], averages become length-B arrays).
 """
 import ctypes
-from types import FunctionType, MethodType
+from types import FunctionType
 
 import numpy as np
 
 from . import rtc
+from ._params import KeywordParams
 from .device import DeviceArray, current_stream, empty, require_gpu, to_device
 from .engine import FWD, INV, Plan, mem_info, rank1_split, sum_partials
 from . import graphs, wm_ops
@@ -31,7 +32,7 @@ from .wm_ops import WMGrid
 CHUNK_STEPS = 256
 
 
-class WignerMoyalCUDA1D:
+class WignerMoyalCUDA1D(KeywordParams):
     """The second-order split-operator propagator for the Moyal equation for the Wigner
     function W(x, p, t) with the time-dependent Hamiltonian H = K(p, t) + V(x, t)
     (reference: wigner_moyal_cuda_1d.py:10-16), on MI355X."""
@@ -46,11 +47,7 @@ class WignerMoyalCUDA1D:
         self.precision = kwargs.pop("precision", "fp64")
         self.batch = int(kwargs.pop("batch", 1))
         # save all attributes; Python functions become bound methods (:44-50)
-        for name, value in kwargs.items():
-            if isinstance(value, FunctionType):
-                setattr(self, name, MethodType(value, self))
-            else:
-                setattr(self, name, value)
+        self._set_params(kwargs)
 
         # required-parameter checks (:53-92)
         if not hasattr(self, "X_gridDIM"):

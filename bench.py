@@ -152,6 +152,7 @@ def main():
     W0 = initial_states(mixed, B)
     mixed.set_wignerfunction(W0)
     wall, pass_ms = time_propagate(mixed, W, K)
+    group = mixed.plan.step_group(min(K, 256), B)
     value = B * N * N * K / wall
     bytes_per_pass = 2 * B * N * N * 8          # one read + one write of the c64 state
     achieved = bytes_per_pass / (pass_ms * 1e-3) / 1e9
@@ -233,7 +234,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "kernel": "k_line_pass (row and column passes)", "avg_pass_ms": pass_ms,
                      "bytes_per_pass": bytes_per_pass, "copy_GBps": copy_gbps,
-                     "frac_of_copy": achieved / copy_gbps},
+                     "frac_of_copy": achieved / copy_gbps,
+                     # a pass over the batch is launched per group of items that stays in the
+                     # MALL between passes: per launch, bytes and time are both 1/launches
+                     "group_items": group, "launches_per_pass": -(-B // group)},
         "latency": {"workload": "config 1 solve: WM FP64 256x256, 1000 steps, norm/<x>/<p>/<H> every step "
                                 "(20 timed solves)",
                     "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),

@@ -117,6 +117,9 @@ int aw_pass_run(aw_plan *plan, void *data, const aw_pass *pass, aw_stream stream
  * const_steps = 1 (time-independent H): the column table holds 3 vectors per item
  * (0 opening, 1 merged, 2 closing) and the row table 1, reused for every step.
  * batch > 0 runs only the first `batch` items of the plan (a slot pool's active prefix).
+ * A batch larger than 192 MiB with n >= 2 is stepped in groups of items of at most that
+ * size, each group through all 2n + 1 passes, so that it stays in the MALL between passes;
+ * the results are bit-identical to stepping the whole batch per pass.
  */
 typedef struct aw_steps {
     int nsteps;
@@ -130,6 +133,14 @@ typedef struct aw_steps {
 } aw_steps;
 
 int aw_step_sequence(aw_plan *plan, void *data, const aw_steps *steps, aw_stream stream);
+
+/*
+ * Items per group that aw_step_sequence uses for `nsteps` steps over the first `batch`
+ * items (0: the plan's batch); equal to the batch when it is not split.  No reference
+ * counterpart (the reference propagates one state): lets a caller relate kernel launches
+ * to passes over its batch.
+ */
+int aw_step_group(const aw_plan *plan, int nsteps, int batch, int *group);
 
 /*
  * Build complex factor vectors in the plan's precision from FP64 inputs:
