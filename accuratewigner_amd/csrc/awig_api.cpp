@@ -20,6 +20,9 @@ struct aw_plan {
     double *partials = nullptr;
     int red_blocks = 0;
     double *diag_work = nullptr;   // cyclic-diagonal partial sums, allocated on first use
+    double *wsum_partials = nullptr;    // weighted-sum block partials, allocated on first use
+    unsigned *wsum_counters = nullptr;  // their per-item tickets (zero between launches)
+    int wsum_blocks = 0;
 };
 
 struct aw_module {
@@ -117,6 +120,8 @@ int aw_plan_destroy(aw_plan *p) {
     if (p->tw_cols) (void)hipFree(p->tw_cols);
     if (p->partials) (void)hipFree(p->partials);
     if (p->diag_work) (void)hipFree(p->diag_work);
+    if (p->wsum_partials) (void)hipFree(p->wsum_partials);
+    if (p->wsum_counters) (void)hipFree(p->wsum_counters);
     delete p;
     return AW_OK;
 }
@@ -331,6 +336,28 @@ int aw_cyclic_diag_sums(aw_plan *p, const void *data, double *out, aw_stream str
         AW_HIP(hipMalloc((void **)&p->diag_work, sizeof(double) * 2 * (size_t)n * nblk * p->batch));
     }
     return launch_cyclic_diag(p->prec, data, n, p->batch, p->diag_work, out, (hipStream_t)stream);
+}
+
+int aw_weighted_sums(aw_plan *p, const void *data, const double *weights, int nw, double *out, double factor,
+                     double *dscale, aw_stream stream) {
+    if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
+    if (!data || !weights || !out) return set_error(AW_ERR_INVALID_VALUE, "NULL data, weights or out");
+    if (nw < 1 || nw > 8) return set_error(AW_ERR_INVALID_VALUE, "nw must be 1..8 (got %d)", nw);
+    const long long n_item = (long long)p->rows * p->cols;
+    if (!p->wsum_partials) {
+        // scratch of the first call: allocation and the counter reset cannot be captured
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        AW_HIP(hipStreamIsCapturing((hipStream_t)stream, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return set_error(AW_ERR_INVALID_VALUE, "the first aw_weighted_sums call on a plan must not be captured");
+        long long nb = n_item / 2048;
+        p->wsum_blocks = int(nb < 1 ? 1 : (nb > kWsumMaxBlocks ? kWsumMaxBlocks : nb));
+        AW_HIP(hipMalloc((void **)&p->wsum_counters, sizeof(unsigned) * (size_t)p->batch));
+        AW_HIP(hipMemsetAsync(p->wsum_counters, 0, sizeof(unsigned) * (size_t)p->batch, (hipStream_t)stream));
+        AW_HIP(hipMalloc((void **)&p->wsum_partials, sizeof(double) * 8 * (size_t)p->wsum_blocks * p->batch));
+    }
+    return launch_weighted_sums(p->prec, data, n_item, weights, nw, p->batch, p->wsum_partials, p->wsum_counters,
+                                p->wsum_blocks, out, factor, dscale, (hipStream_t)stream);
 }
 
 int aw_sum_partials(const double *partials, int nblk, int ncomp, int batch, double *out,

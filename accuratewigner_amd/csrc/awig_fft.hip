@@ -204,6 +204,93 @@ int launch_pair_dot(int prec, const void *a, const void *b, long long n_item, in
     return AW_OK;
 }
 
+// Real parts of weighted sums of each item: out[b][q] = Re sum_i conj(w[q][i]) z[b][i] for
+// q < NW, FP64 weights shared by the items.  Block partials are added by the last block of
+// each item to finish (ticket counter), in block order, so one launch gives deterministic
+// sums; that block also writes the normalisation scale 1 / (out[b][0] factor) when dscale is
+// given and resets the item's counter for the next launch (replayable from a graph).
+template <typename R, int NW>
+__global__ __launch_bounds__(RED_THREADS) void k_weighted_sums(const void *data, long long n_item,
+                                                                 const double2 *w, double *partials,
+                                                                 unsigned *counters, double *out,
+                                                                 double factor, double *dscale) {
+    using T2 = typename Cx<R>::T;
+    const int b = blockIdx.y;
+    const int nblk = gridDim.x;
+    const T2 *z = reinterpret_cast<const T2 *>(data) + (long long)b * n_item;
+    double acc[NW];
+#pragma unroll
+    for (int q = 0; q < NW; ++q) acc[q] = 0.0;
+    const long long stride = (long long)nblk * RED_THREADS;
+    for (long long i = (long long)blockIdx.x * RED_THREADS + threadIdx.x; i < n_item; i += stride) {
+        const T2 v = z[i];
+        const double x = v.x, y = v.y;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+            const double2 c = w[q * n_item + i];
+            acc[q] += c.x * x + c.y * y;
+        }
+    }
+    double *pb = partials + (long long)b * nblk * NW;
+    block_sum_store<NW>(acc, pb + (long long)blockIdx.x * NW);
+    // the partials of this block are stored; the last block of the item adds them all
+    __shared__ int last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(counters + b, 1u) == unsigned(nblk - 1);
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x < NW) {
+        double s = 0.0;
+        for (int k = 0; k < nblk; ++k) s += pb[(long long)k * NW + threadIdx.x];
+        out[(long long)b * NW + threadIdx.x] = s;
+        if (threadIdx.x == 0 && dscale) {
+            const double re = s * factor, im = 0.0;
+            const double d = re * re + im * im;
+            dscale[2 * b] = re / d;
+            dscale[2 * b + 1] = -im / d;
+        }
+    }
+    if (threadIdx.x == 0) atomicExch(counters + b, 0u);
+}
+
+template <typename R, int NW>
+static void launch_wsums_t(const void *data, long long n_item, const double *w, int batch, double *partials,
+                           unsigned *counters, int nblk, double *out, double factor, double *dscale,
+                           hipStream_t st) {
+    hipLaunchKernelGGL((k_weighted_sums<R, NW>), dim3(nblk, batch), dim3(RED_THREADS), 0, st, data, n_item,
+                       reinterpret_cast<const double2 *>(w), partials, counters, out, factor, dscale);
+}
+
+template <typename R>
+static int launch_wsums_r(const void *data, long long n_item, const double *w, int nw, int batch,
+                          double *partials, unsigned *counters, int nblk, double *out, double factor,
+                          double *dscale, hipStream_t st) {
+#define AW_WS(K) \
+    case K: launch_wsums_t<R, K>(data, n_item, w, batch, partials, counters, nblk, out, factor, dscale, st); break;
+    switch (nw) {
+        AW_WS(1) AW_WS(2) AW_WS(3) AW_WS(4) AW_WS(5) AW_WS(6) AW_WS(7) AW_WS(8)
+    default: return set_error(AW_ERR_INVALID_VALUE, "nw must be 1..8 (got %d)", nw);
+    }
+#undef AW_WS
+    return AW_OK;
+}
+
+int launch_weighted_sums(int prec, const void *data, long long n_item, const double *w, int nw, int batch,
+                         double *partials, unsigned *counters, int nblk, double *out, double factor,
+                         double *dscale, hipStream_t st) {
+    const int rc = prec == AW_PREC_F64
+                       ? launch_wsums_r<double>(data, n_item, w, nw, batch, partials, counters, nblk, out,
+                                                factor, dscale, st)
+                       : launch_wsums_r<float>(data, n_item, w, nw, batch, partials, counters, nblk, out,
+                                               factor, dscale, st);
+    if (rc != AW_OK) return rc;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_weighted_sums: %s", hipGetErrorString(e));
+    return AW_OK;
+}
+
 // signed cyclic diagonal sums of square items: out[b][d] = (-1)^d sum_i z[i][(i + d) mod n]
 // (complex FP64).  For fixed i the threads d, d+1, ... read consecutive columns of row i.
 template <typename R>

@@ -54,6 +54,10 @@ int launch_pair_dot(int prec, const void *a, const void *b, long long n_item, in
                     const double *v, int batch, double *partials, int nblk, double *out, hipStream_t st);
 constexpr int kDiagRowsPerBlock = 16;
 int launch_cyclic_diag(int prec, const void *data, int n, int batch, double *work, double *out, hipStream_t st);
+constexpr int kWsumMaxBlocks = 512;
+int launch_weighted_sums(int prec, const void *data, long long n_item, const double *w, int nw, int batch,
+                         double *partials, unsigned *counters, int nblk, double *out, double factor,
+                         double *dscale, hipStream_t st);
 int launch_sum_partials(const double *partials, int nblk, int ncomp, int batch, double *out,
                         hipStream_t st);
 int launch_scale_from_stats(const double *stats, int stride, int comp, int mode, double factor,

@@ -154,6 +154,15 @@ class Plan:
                   ctypes.c_void_p(current_stream()))
         return out
 
+    def weighted_sums(self, data, weights, nw, out, factor=0.0, dscale=None):
+        """out[batch, nw] (device FP64) = Re sum conj(weights[q]) data per item (weights:
+        complex128 [nw, rows, cols]); with dscale, also dscale = 1 / (out[:, 0] factor).  The
+        first call on a plan allocates its scratch (never under graph capture)."""
+        _lib.call("aw_weighted_sums", self.handle, ctypes.c_void_p(_p(data)), ctypes.c_void_p(_p(weights)),
+                  int(nw), ctypes.c_void_p(_p(out)), float(factor), ctypes.c_void_p(_p(dscale)),
+                  ctypes.c_void_p(current_stream()))
+        return out
+
     def cyclic_diag_sums(self, data, out):
         """out[batch, n] complex128 (device): (-1)^d sum_i data[i, (i + d) mod n]."""
         _lib.call("aw_cyclic_diag_sums", self.handle, ctypes.c_void_p(_p(data)), ctypes.c_void_p(_p(out)),

@@ -314,13 +314,13 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
             p.steps(rho, 1, colL.ptr + 2 * k * N * csz, colE.ptr + 2 * k * N * csz,
                     rowL.ptr + k * N * csz, rowE.ptr + k * N * csz, dscale=self._dscale)
             self._scale_pending = False
-            self._diag_obs_ptr(xw.ptr + k * 3 * N * 8, 3, stage.ptr + k * 64)
+            # trace and x-observables, and the next scale 1 / (Re tr(rho) dX)
+            self._diag_obs_ptr(xw.ptr + k * 3 * N * 8, 3, stage.ptr + k * 64, self.dX, self._dscale)
             p.cyclic_diag_sums(rho, bufs["csum"])
             self._cplan.fft(bufs["csum"], 1, FWD, out=bufs["chat"])
             pmod.launch("aw_p_obs", (1,), (256,), [bufs["chat"].ptr, pw.ptr + k * 3 * N * 8, ctypes.c_int(3),
                                                    ctypes.c_int(N), stage.ptr + (k * 8 + 4) * 8],
                         current_stream())
-            p.scale_from_stats(stage.ptr + k * 64, 8, 0, 0, self.dX, self._dscale)
             self._scale_pending = True
 
     def single_step_propagation(self):
@@ -419,9 +419,11 @@ extern "C" __global__ void aw_weights_t(const double *Xg, const double *Pg, int 
         double *out = out_all + (long long)s * NOBS_W * n;
 %s    }
 }
-extern "C" __global__ void aw_diag_obs(const %s *rho, const double *w, int nobs, int n, double *out)
+extern "C" __global__ void aw_diag_obs(const %s *rho, const double *w, int nobs, int n, double *out,
+                                       double factor, double *dscale)
 {
-    // out[0] = Re tr rho, out[1 + q] = sum_j Re rho_jj w_q[j]  (one block)
+    // out[0] = Re tr rho, out[1 + q] = sum_j Re rho_jj w_q[j]  (one block); with dscale, also
+    // the normalisation scale 1 / (out[0] factor) (the formula of libawig's k_scale_from_stats)
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
         const double d = (double)rho[(long long)k * n + k].x;
@@ -439,6 +441,12 @@ extern "C" __global__ void aw_diag_obs(const %s *rho, const double *w, int nobs,
         double s = 0.;
         for (int q = 0; q < 4; ++q) s += red[q][threadIdx.x];
         if (threadIdx.x <= nobs) out[threadIdx.x] = s;
+        if (threadIdx.x == 0 && dscale) {
+            const double re = s * factor, im = 0.;
+            const double d = re * re + im * im;
+            dscale[0] = re / d;
+            dscale[1] = -im / d;
+        }
     }
 }
 extern "C" __global__ void aw_copy_scalar(const double *src, double *dst) { *dst = *src; }
@@ -482,10 +490,13 @@ extern "C" __global__ void aw_p_obs(const double2 *chat, const double *w, int no
     def _diag_obs(self, w, nobs, out_ptr):
         self._diag_obs_ptr(w.ptr, nobs, out_ptr)
 
-    def _diag_obs_ptr(self, w_ptr, nobs, out_ptr):
+    def _diag_obs_ptr(self, w_ptr, nobs, out_ptr, factor=0.0, dscale=None):
+        """out[0] = Re tr rho, out[1 + q] = sum_j Re rho_jj w_q[j]; with dscale, also the
+        normalisation scale 1 / (out[0] factor)."""
         mod = self._obs_module(["X"])
         mod.launch("aw_diag_obs", (1,), (256,), [self.rho.ptr, w_ptr, ctypes.c_int(nobs),
-                                                  ctypes.c_int(self.X_gridDIM), out_ptr],
+                                                  ctypes.c_int(self.X_gridDIM), out_ptr, ctypes.c_double(factor),
+                                                  ctypes.c_void_p(dscale.ptr if dscale is not None else None)],
                    current_stream())
 
     def _weights_t(self, exprs, ts_dev, nt, out):

@@ -18,6 +18,7 @@ arguments and FP64 reductions); ``batch`` = B > 1 propagates B independent state
 same Hamiltonian in every launch (arrays become [B, Np, Nx], averages become length-B arrays).
 """
 import ctypes
+import re
 from types import FunctionType
 
 import numpy as np
@@ -31,13 +32,30 @@ from .wm_ops import WMGrid
 
 CHUNK_STEPS = 256
 
+_T_TOKEN = re.compile(r"\bt\b")
+
+
+def time_independent(expr, functions=""):
+    """True when the C expression cannot depend on t: it names no ``t`` and the user's
+    helper code defines no macro that does (a helper function only sees t through its
+    arguments).  Conservative: a false negative only keeps the per-step Wigner transform."""
+    if _T_TOKEN.search(expr):
+        return False
+    for line in (functions or "").replace("\\\n", " ").splitlines():
+        if line.strip().startswith("#define") and _T_TOKEN.search(line):
+            return False
+    return True
+
 
 class WignerMoyalCUDA1D(KeywordParams):
     """The second-order split-operator propagator for the Moyal equation for the Wigner
     function W(x, p, t) with the time-dependent Hamiltonian H = K(p, t) + V(x, t)
     (reference: wigner_moyal_cuda_1d.py:10-16), on MI355X."""
 
-    # which C-expression observables the Ehrenfest path records (get_Ehrenfest, :584-606)
+    # record time-independent Ehrenfest observables as Re <L^H f, rho> (one reduction per step)
+    # instead of forming W each step; False keeps the Wigner transform every step
+    obs_by_adjoint = True
+
     def __init__(self, **kwargs):
         """Keywords as in the reference constructor (wigner_moyal_cuda_1d.py:17-41):
         X_gridDIM, X_amplitude, P_gridDIM, P_amplitude, V, K, dt, and optionally t, consts,
@@ -221,6 +239,8 @@ extern "C" __global__ void aw_eval_mask(int which, const double *xj, int nj, con
         if mod is None:
             nobs = len(exprs)
             body = "".join("        acc[%d] += w * (%s);\n" % (k, e) for k, e in enumerate(exprs))
+            grid_body = "".join("        out[%d * (long long)n_item + idx] = make_double2((%s), 0.);\n" % (k, e)
+                                for k, e in enumerate(exprs))
             src = (rtc.PRELUDE + self.cuda_consts
                    + "\n__device__ double K(double P, double t)\n{\n    return (%s);\n}\n" % self.K
                    + "\n__device__ double V(double X, double t)\n{\n    return (%s);\n}\n" % self.V
@@ -236,40 +256,82 @@ __device__ __forceinline__ void aw_obs_body(const %(st)s *W, int n_item, double 
         const double P = dP * (i - 0.5 * P_gridDIM);
         const double w = (double)Wb[idx].x;
 %(body)s    }
-""" % dict(st=self._state_t(), nobs=nobs, body=body) + rtc.REDUCE_TAIL % dict(nobs=nobs) + r"""}
+""" % dict(st=self._state_t(), nobs=nobs, body=body) + rtc.REDUCE_TAIL % dict(nobs=nobs) + "}\n"
+                   + rtc.REDUCE_FINISH % dict(nobs=nobs) + r"""
 extern "C" __global__ void aw_obs(const %(st)s *W, int n_item, double t, double *partials)
 {
     aw_obs_body(W, n_item, t, partials);
 }
 // the same with the time read from device memory (a captured step sequence replays with
-// new times written into the buffer)
-extern "C" __global__ void aw_obs_tp(const %(st)s *W, int n_item, const double *tp, double *partials)
+// new times written into the buffer), summed into out[item][:] by the last block, which
+// also writes the normalisation scale 1 / (out[item][0] * factor)
+extern "C" __global__ void aw_obs_fin(const %(st)s *W, int n_item, const double *tp, double *partials,
+                                      unsigned *counters, double *out, double factor, double *dscale)
 {
     aw_obs_body(W, n_item, *tp, partials);
+    aw_finish(partials, counters, out, factor, dscale);
 }
-""" % dict(st=self._state_t()))
+// the expressions themselves on the grid at time t: out[k][i][j] = (f_k(X_j, P_i, t), 0)
+extern "C" __global__ void aw_obs_grid(double2 *out, int n_item, double t)
+{
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n_item; idx += gridDim.x * blockDim.x) {
+        const int i = idx / X_gridDIM, j = idx - i * X_gridDIM;
+        const double X = dX * (j - 0.5 * X_gridDIM);
+        const double P = dP * (i - 0.5 * P_gridDIM);
+%(grid_body)s    }
+}
+""" % dict(st=self._state_t(), grid_body=grid_body))
             mod = rtc.Module(src, "wigner_moyal_observables.hip")
             self._observable_kernels[key] = mod
         return mod
 
-    def _obs_launch(self, exprs, W, t, out, t_ptr=None):
-        """Partial sums of Re W * f_k(X, P, t) for each expression, summed into out[k]
-        (t_ptr: a device double holding t instead of the value)."""
-        mod = self._observable_kernel(exprs)
+    def _obs_partials_for(self, nexprs):
+        """(blocks per item, partial-sum buffer) of the observable kernels; the buffer is
+        reused by every launch of this shape (stream order keeps the uses apart)."""
         n = self.X_gridDIM * self.P_gridDIM
         nblk = max(1, min(512, n // 2048))
-        key = (nblk, len(exprs))
+        key = (nblk, nexprs)
         partials = self._obs_partials.get(key)
         if partials is None:
-            # reused by every launch of this shape (stream order keeps the uses apart)
-            partials = self._obs_partials[key] = empty((self.batch, nblk, len(exprs)), np.float64)
-        if t_ptr is None:
-            mod.launch("aw_obs", (nblk, self.batch), (256,), [W.ptr, ctypes.c_int(n), ctypes.c_double(t),
-                                                              partials.ptr], current_stream())
-        else:
-            mod.launch("aw_obs_tp", (nblk, self.batch), (256,), [W.ptr, ctypes.c_int(n), t_ptr,
-                                                                 partials.ptr], current_stream())
+            partials = self._obs_partials[key] = empty((self.batch, nblk, nexprs), np.float64)
+        return nblk, partials
+
+    def _obs_launch(self, exprs, W, t, out):
+        """Partial sums of Re W * f_k(X, P, t) for each expression, summed into out[k]."""
+        mod = self._observable_kernel(exprs)
+        n = self.X_gridDIM * self.P_gridDIM
+        nblk, partials = self._obs_partials_for(len(exprs))
+        mod.launch("aw_obs", (nblk, self.batch), (256,), [W.ptr, ctypes.c_int(n), ctypes.c_double(t),
+                                                          partials.ptr], current_stream())
         sum_partials(partials, nblk, len(exprs), self.batch, out)
+
+    def _obs_fin_launch(self, exprs, W, t_ptr, out_ptr, counters, factor, dscale):
+        """One launch: the sums of Re W * f_k(X, P, t) (t read from device memory at t_ptr)
+        into out[item][k] and the normalisation scale 1 / (out[item][0] factor) into dscale."""
+        mod = self._observable_kernel(exprs)
+        n = self.X_gridDIM * self.P_gridDIM
+        nblk, partials = self._obs_partials_for(len(exprs))
+        mod.launch("aw_obs_fin", (nblk, self.batch), (256,),
+                   [W.ptr, ctypes.c_int(n), t_ptr, partials.ptr, counters.ptr, out_ptr,
+                    ctypes.c_double(factor), dscale.ptr], current_stream())
+
+    def _obs_weights(self, exprs):
+        """g_k = L^H f_k for observables f_k(X, P) that do not depend on t, with L = rho2wigner_
+        (the unnormalised transform of wigner_moyal_cuda_1d.py:471-501): then
+        sum Re(W) f_k = Re <f_k, L rho> = Re <g_k, rho>, one weighted sum over rho per step
+        instead of the Wigner transform and the observable pass.  complex128 [nobs, Np, Nx],
+        or None when an expression may depend on t (or there are more than 8)."""
+        if len(exprs) > 8 or not all(time_independent(e, getattr(self, "functions", "")) for e in exprs):
+            return None
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        mod = self._observable_kernel(exprs)
+        g = empty((len(exprs), Np, Nx), np.complex128)
+        blocks, threads = rtc.launch_cfg(Nx * Np)
+        mod.launch("aw_obs_grid", (blocks,), (threads,), [g.ptr, ctypes.c_int(Nx * Np), ctypes.c_double(0.0)],
+                   current_stream())
+        self._obs_plan = Plan(Np, Nx, len(exprs), "fp64")   # lives with the system (its launches are queued)
+        wm_ops.rho2wigner_adjoint(self._obs_plan, self._grid, g)
+        return g
 
     def _per_item(self, values):
         values = np.asarray(values, dtype=np.float64)
@@ -573,9 +635,14 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         CH = CHUNK_STEPS
         if self._ehr is None:
             self._ehr = dict(self._chunk_bufs(CH), stage=empty((CH, self.batch, nobs), np.float64),
-                             graphs={})
+                             counters=to_device(np.zeros(self.batch, np.int32)), graphs={})
             self._observable_kernel(exprs)
-            self._obs_launch(exprs, self.wignerfunction, 0.0, self._ehr["stage"])   # allocates partials
+            self._obs_partials_for(nobs)             # allocated here, never inside a capture
+            g = self._obs_weights(exprs) if self.obs_by_adjoint else None
+            self._ehr["weights"] = g
+            if g is not None:
+                # the first call allocates the plan's scratch (never inside a capture)
+                self.plan.weighted_sums(self.rho, g, nobs, self._ehr["stage"])
         bufs = self._ehr
         if not self._scale_pending:
             # every step applies the pending scale: start from 1 so that all chunks are alike
@@ -591,8 +658,11 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             ring.tensor[c0:c0 + n].copy_(bufs["stage"].tensor[:n])
         self.t = ts[-1]
         # final normalisation of rho and W by the last step's norm
-        self.plan.scale(self.wignerfunction, self._dscale)
-        self._flush_scale()
+        if bufs["weights"] is not None:
+            self.rho2wigner_()           # W of the normalised rho (not formed during the steps)
+        else:
+            self.plan.scale(self.wignerfunction, self._dscale)
+            self._flush_scale()
         r = ring.get()
         s0 = r[:, :, 0]
         for lst, k, sgn in ((self.X_average, 1, 1.), (self.X_average_RHS, 2, 1.),
@@ -624,15 +694,22 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         nobs = len(exprs)
         row_bytes = self.batch * nobs * 8
         stage, ts = bufs["stage"], bufs["ts"]
+        g = bufs["weights"]
         for k in range(n):
             # one step = three passes; its opening / closing vectors sit at 2 k and 2 k + 1
             p.steps(rho, 1, colL.ptr + 2 * k * Nx * csz, colE.ptr + 2 * k * Np * csz,
                     rowL.ptr + k * Np * csz, rowE.ptr + k * Nx * csz, mask_open_close=tab1,
                     mask_row=tabp, dscale=self._dscale)
             self._scale_pending = False
+            if g is not None:
+                # time-independent observables: Re <g_k, rho> and the next scale, one launch
+                p.weighted_sums(rho, g, nobs, stage.ptr + k * row_bytes, self.dXdP, self._dscale)
+                self._scale_pending = True
+                continue
             self.rho2wigner_()
-            self._obs_launch(exprs, self.wignerfunction, 0.0, stage.ptr + k * row_bytes, t_ptr=ts.ptr + 8 * k)
-            p.scale_from_stats(stage.ptr + k * row_bytes, nobs, 0, 0, self.dXdP, self._dscale)
+            # observables into the stage row and the next scale 1 / (norm dXdP), one launch
+            self._obs_fin_launch(exprs, self.wignerfunction, ts.ptr + 8 * k, stage.ptr + k * row_bytes,
+                                 bufs["counters"], self.dXdP, self._dscale)
             self._scale_pending = True
 
     def get_Ehrenfest(self, t):

@@ -152,10 +152,15 @@ def main():
     W0 = initial_states(mixed, B)
     mixed.set_wignerfunction(W0)
     wall, pass_ms = time_propagate(mixed, W, K)
-    group = mixed.plan.step_group(min(K, 256), B)
     value = B * N * N * K / wall
+    # a pass over the batch is one launch per group of items (groups stay in the MALL
+    # between passes; equal groups here): per launch, bytes and time are both 1/launches
+    group = mixed.plan.step_group(min(K, 256), B)
+    launches = -(-B // group)
     bytes_per_pass = 2 * B * N * N * 8          # one read + one write of the c64 state
-    achieved = bytes_per_pass / (pass_ms * 1e-3) / 1e9
+    bytes_per_launch = 2 * group * N * N * 8
+    launch_ms = pass_ms * group / B
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
 
     # -- plain FP64 path on the first Bf states ---------------------------------------------
     Bf = min(args.fp64_batch, B)
@@ -206,7 +211,7 @@ def main():
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_wm1024_mixed_b%d.json" % B)
     if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("hbm_bytes_per_pass")
+        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu:
@@ -232,12 +237,11 @@ def main():
         "rel_l2_err_vs_fp64": err, "err_steps": W + K,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "kernel": "k_line_pass (row and column passes)", "avg_pass_ms": pass_ms,
+                     "kernel": "k_line_pass (row and column passes)", "avg_launch_ms": launch_ms,
+                     "bytes_per_launch": bytes_per_launch, "group_items": group,
+                     "launches_per_pass": launches, "avg_pass_ms": pass_ms,
                      "bytes_per_pass": bytes_per_pass, "copy_GBps": copy_gbps,
-                     "frac_of_copy": achieved / copy_gbps,
-                     # a pass over the batch is launched per group of items that stays in the
-                     # MALL between passes: per launch, bytes and time are both 1/launches
-                     "group_items": group, "launches_per_pass": -(-B // group)},
+                     "frac_of_copy": achieved / copy_gbps},
         "latency": {"workload": "config 1 solve: WM FP64 256x256, 1000 steps, norm/<x>/<p>/<H> every step "
                                 "(20 timed solves)",
                     "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),

@@ -179,6 +179,21 @@ int aw_pair_dot(aw_plan *plan, const void *a, const void *b, const double *u, co
                 double *out, aw_stream stream);
 
 /*
+ * Real parts of weighted sums of each batch item (FP64 accumulation, fixed order):
+ *   out[b][q] = Re sum_ij conj(w[q][i][j]) data[b][i][j],   q < nw (1..8),
+ * weights complex128 [nw][rows][cols], shared by every item; out [batch][nw] device FP64.
+ * With dscale != NULL also dscale[b] = 1 / (out[b][0] * factor) (complex, imaginary part 0),
+ * the normalisation of the next step.  One launch (the last block of each item adds the
+ * block partials).  The propagators record the reference's per-step Ehrenfest averages
+ * sum Re W f_q dXdP (wigner_moyal_cuda_1d.py:584-606, W from rho2wigner_ :471-501) as
+ * Re <g_q, rho> with g_q = rho2wigner^H f_q when f_q does not depend on t.  The first call
+ * on a plan allocates its scratch and must not be made under stream capture; calls on one
+ * plan must not overlap.
+ */
+int aw_weighted_sums(aw_plan *plan, const void *data, const double *weights, int nw, double *out,
+                     double factor, double *dscale, aw_stream stream);
+
+/*
  * Signed cyclic diagonal sums of square items (n >= 64), FP64 complex:
  *   out[b][d] = (-1)^d sum_i data[b][i][(i + d) mod n],  d = 0..n-1   (2 n doubles per item).
  * The forward DFT of this vector is the momentum-space diagonal of rho scaled by n, so a

@@ -1,0 +1,66 @@
+"""Ehrenfest averages of time-independent observables recorded as Re <L^H f, rho> (one
+weighted sum per step, L = the Wigner transform) against the same run that forms W every
+step and sums Re(W) f; time-dependent observables keep the per-step transform."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LISTS = ("X_average", "P_average", "X_average_RHS", "P_average_RHS", "hamiltonian_average")
+
+
+def _run(by_adjoint, precision="fp64", batch=1, steps=300, V="0.5 * X * X + 0.1 * X * X * X * X",
+         diff_V="X + 0.4 * X * X * X", mask=None):
+    from accuratewigner_amd import WignerMoyalCUDA1D
+    kw = dict(abs_boundary_x=mask) if mask else {}
+    s = WignerMoyalCUDA1D(X_gridDIM=128, X_amplitude=8., P_gridDIM=128, P_amplitude=8., dt=0.01, t=0.,
+                          K="0.5 * P * P", V=V, diff_K="P", diff_V=diff_V, precision=precision,
+                          batch=batch, **kw)
+    s.obs_by_adjoint = by_adjoint
+    X, P = s.X, s.P
+    x0 = np.array([1.0, -0.5, 0.3])[:batch]
+    W0 = np.exp(-(X[None] - x0[:, None, None]) ** 2 - (P[None] - 0.2) ** 2 / 0.8) / np.pi
+    s.set_wignerfunction(W0 if batch > 1 else W0[0])
+    s.propagate(steps)
+    return s
+
+
+@pytest.mark.parametrize("mask", [None, "exp(-pow(XX / 10., 20)) * exp(-pow(XX_prime / 10., 20))"])
+def test_adjoint_weights_match_the_per_step_transform_fp64(mask):
+    a, b = _run(True, mask=mask), _run(False, mask=mask)
+    assert a._ehr["weights"] is not None and b._ehr["weights"] is None
+    for name in LISTS:
+        x, y = np.array(getattr(a, name)), np.array(getattr(b, name))
+        assert x.shape == (300,)
+        assert np.max(np.abs(x - y)) < 1e-12 * max(1.0, np.max(np.abs(y))), name
+    wa, wb = a.wignerfunction.get(), b.wignerfunction.get()
+    assert np.linalg.norm(wa - wb) < 1e-12 * np.linalg.norm(wb)
+    # a second call continues from the same state (captured chunk replayed)
+    a.propagate(40)
+    b.propagate(40)
+    for name in LISTS:
+        x, y = np.array(getattr(a, name)), np.array(getattr(b, name))
+        assert np.max(np.abs(x - y)) < 1e-12 * max(1.0, np.max(np.abs(y))), name
+
+
+def test_adjoint_weights_batch_mixed():
+    """Mixed precision, three states with their own norms: both ways of recording stay within
+    the mixed tier's budget of the FP64 run.  (The adjoint way measures the complex64 rho in
+    FP64; the per-step transform adds the rounding of four FP32 passes, which may happen to
+    cancel part of the propagation error, so neither is always the closer.)"""
+    ref = _run(True, "fp64", batch=3)
+    a, b = _run(True, "mixed", batch=3), _run(False, "mixed", batch=3)
+    for name in LISTS:
+        r = np.array(getattr(ref, name))
+        x, y = np.array(getattr(a, name)), np.array(getattr(b, name))
+        assert x.shape == (300, 3)
+        scale = max(1.0, np.max(np.abs(r)))
+        ea, eb = np.max(np.abs(x - r)) / scale, np.max(np.abs(y - r)) / scale
+        print("%s: adjoint weights %.2e, per-step transform %.2e" % (name, ea, eb))
+        assert ea < 1e-4 and eb < 1e-4, name
+
+
+def test_time_dependent_observables_keep_the_transform():
+    s = _run(True, V="0.5 * X * X + X * 0.1 * sin(0.5 * t)", diff_V="X + 0.1 * sin(0.5 * t)", steps=20)
+    assert s._ehr["weights"] is None
+    assert len(s.X_average) == 20

@@ -60,3 +60,16 @@ def test_plans_dropped_during_a_capture_are_destroyed_after_it(monkeypatch):
     capturing[0] = False
     engine.destroy_deferred_plans()
     assert destroyed == [12345] and engine._DEFERRED_PLANS == []
+
+
+def test_time_independence_rule_for_observables():
+    from accuratewigner_amd.wigner_moyal_cuda_1d import time_independent
+    assert time_independent("0.5 * P * P + 0.5 * X * X")
+    assert time_independent("1.") and time_independent("X / pow(X * X + 1.37, 1.5)")
+    assert not time_independent("X * E(t)")
+    assert not time_independent("sin(omega * t) * X")
+    assert time_independent("tanh(X) + t_0", "")              # identifiers that merely contain t
+    # a helper macro that reads t makes every observable suspect
+    assert not time_independent("X * Et", "#define Et (F * sin(omega * t))")
+    assert not time_independent("X * Et", "#define Et \\\n    (F * sin(omega * t))")   # continued line
+    assert time_independent("X * E0", "__device__ double E(double t) { return t; }")
