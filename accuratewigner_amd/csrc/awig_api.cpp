@@ -22,7 +22,6 @@ struct aw_plan {
     double *diag_work = nullptr;   // cyclic-diagonal partial sums, allocated on first use
     double *wsum_partials = nullptr;    // weighted-sum block partials, allocated on first use
     unsigned *wsum_counters = nullptr;  // their per-item tickets (zero between launches)
-    int wsum_blocks = 0;
 };
 
 struct aw_module {
@@ -338,26 +337,59 @@ int aw_cyclic_diag_sums(aw_plan *p, const void *data, double *out, aw_stream str
     return launch_cyclic_diag(p->prec, data, n, p->batch, p->diag_work, out, (hipStream_t)stream);
 }
 
+// Scratch of the one-launch reductions (block partials and per-item ticket counters, which
+// start at zero and are reset by each launch's last block), allocated by the first call on a
+// plan.  Allocation and the counter reset cannot be captured into a graph, so that first call
+// must run eagerly.
+static int ensure_finish_scratch(aw_plan *p, aw_stream stream, const char *who) {
+    if (p->wsum_partials) return AW_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    AW_HIP(hipStreamIsCapturing((hipStream_t)stream, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+        return set_error(AW_ERR_INVALID_VALUE, "the first %s call on a plan must not be captured", who);
+    AW_HIP(hipMalloc((void **)&p->wsum_counters, sizeof(unsigned) * (size_t)p->batch));
+    AW_HIP(hipMemsetAsync(p->wsum_counters, 0, sizeof(unsigned) * (size_t)p->batch, (hipStream_t)stream));
+    AW_HIP(hipMalloc((void **)&p->wsum_partials, sizeof(double) * 8 * (size_t)kWsumMaxBlocks * p->batch));
+    return AW_OK;
+}
+
 int aw_weighted_sums(aw_plan *p, const void *data, const double *weights, int nw, double *out, double factor,
                      double *dscale, aw_stream stream) {
     if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
     if (!data || !weights || !out) return set_error(AW_ERR_INVALID_VALUE, "NULL data, weights or out");
     if (nw < 1 || nw > 8) return set_error(AW_ERR_INVALID_VALUE, "nw must be 1..8 (got %d)", nw);
+    const int rc = ensure_finish_scratch(p, stream, "aw_weighted_sums");
+    if (rc != AW_OK) return rc;
     const long long n_item = (long long)p->rows * p->cols;
-    if (!p->wsum_partials) {
-        // scratch of the first call: allocation and the counter reset cannot be captured
+    const long long nb = n_item / 2048;
+    const int nblk = int(nb < 1 ? 1 : (nb > kWsumMaxBlocks ? kWsumMaxBlocks : nb));
+    return launch_weighted_sums(p->prec, data, n_item, weights, nw, p->batch, p->wsum_partials, p->wsum_counters,
+                                nblk, out, factor, dscale, (hipStream_t)stream);
+}
+
+int aw_cyclic_diag_obs(aw_plan *p, const void *data, const double *xw, int nx, const double *pw_hat, int np,
+                       double *out, long long out_bstride, double factor, double *dscale, aw_stream stream) {
+    if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
+    if (!data || !out || (nx > 0 && !xw) || (np > 0 && !pw_hat))
+        return set_error(AW_ERR_INVALID_VALUE, "NULL data, weights or out");
+    if (nx < 0 || np < 0 || 1 + nx + np > 8)
+        return set_error(AW_ERR_INVALID_VALUE, "1 + nx + np must be at most 8 (nx %d, np %d)", nx, np);
+    if (p->rows != p->cols || p->rows < 64)
+        return set_error(AW_ERR_INVALID_SIZE, "cyclic diagonal sums need a square plan with n >= 64");
+    if (out_bstride < 1 + nx + np) return set_error(AW_ERR_INVALID_VALUE, "out_bstride below 1 + nx + np");
+    const int n = p->rows;
+    if (!p->diag_work) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         AW_HIP(hipStreamIsCapturing((hipStream_t)stream, &cs));
         if (cs != hipStreamCaptureStatusNone)
-            return set_error(AW_ERR_INVALID_VALUE, "the first aw_weighted_sums call on a plan must not be captured");
-        long long nb = n_item / 2048;
-        p->wsum_blocks = int(nb < 1 ? 1 : (nb > kWsumMaxBlocks ? kWsumMaxBlocks : nb));
-        AW_HIP(hipMalloc((void **)&p->wsum_counters, sizeof(unsigned) * (size_t)p->batch));
-        AW_HIP(hipMemsetAsync(p->wsum_counters, 0, sizeof(unsigned) * (size_t)p->batch, (hipStream_t)stream));
-        AW_HIP(hipMalloc((void **)&p->wsum_partials, sizeof(double) * 8 * (size_t)p->wsum_blocks * p->batch));
+            return set_error(AW_ERR_INVALID_VALUE, "the first aw_cyclic_diag_obs call on a plan must not be captured");
+        AW_HIP(hipMalloc((void **)&p->diag_work,
+                         sizeof(double) * 2 * (size_t)n * (n / kDiagRowsPerBlock) * p->batch));
     }
-    return launch_weighted_sums(p->prec, data, n_item, weights, nw, p->batch, p->wsum_partials, p->wsum_counters,
-                                p->wsum_blocks, out, factor, dscale, (hipStream_t)stream);
+    const int rc = ensure_finish_scratch(p, stream, "aw_cyclic_diag_obs");
+    if (rc != AW_OK) return rc;
+    return launch_cyclic_obs(p->prec, data, n, p->batch, p->diag_work, xw, nx, pw_hat, np, p->wsum_partials,
+                             p->wsum_counters, out, out_bstride, factor, dscale, (hipStream_t)stream);
 }
 
 int aw_sum_partials(const double *partials, int nblk, int ncomp, int batch, double *out,

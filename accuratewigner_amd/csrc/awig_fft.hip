@@ -363,6 +363,84 @@ int launch_cyclic_diag(int prec, const void *data, int n, int batch, double *wor
     return AW_OK;
 }
 
+// Per-step Ehrenfest sums of square items from the row-block partials of k_cyclic_diag:
+// thread d forms c_d (fixed order over the row blocks) and adds Re(c_d pw_hat[q][d]) for the
+// momentum weights, and Re rho_dd (times xw[q][d]) for the trace and the coordinate weights.
+// Block partials go to `partials`; the last block of the item (ticket counter, reset for the
+// next launch) adds them in block order into out[b][0 .. 1 + nx + np), divides the momentum
+// terms by n, and writes the normalisation scale 1 / (out[b][0] factor) when dscale is given.
+template <typename R>
+__global__ __launch_bounds__(256) void k_cyclic_obs(const void *data, int n, const double *work, int nrb,
+                                                    const double *xw, int nx, const double2 *pw_hat, int np,
+                                                    double *partials, unsigned *counters, double *out,
+                                                    long long out_bstride, double factor, double *dscale) {
+    using T2 = typename Cx<R>::T;
+    const int b = blockIdx.y;
+    const int d = blockIdx.x * 256 + threadIdx.x;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (d < n) {
+        const double *w = work + (long long)b * nrb * 2 * n + 2 * d;
+        double cr = 0.0, ci = 0.0;
+        for (int r = 0; r < nrb; ++r) {
+            cr += w[(long long)r * 2 * n];
+            ci += w[(long long)r * 2 * n + 1];
+        }
+        for (int q = 0; q < np; ++q) {
+            const double2 h = pw_hat[(long long)q * n + d];
+            acc[1 + nx + q] = cr * h.x - ci * h.y;
+        }
+        const T2 z = reinterpret_cast<const T2 *>(data)[(long long)b * n * n + (long long)d * n + d];
+        const double rd = z.x;
+        acc[0] = rd;
+        for (int q = 0; q < nx; ++q) acc[1 + q] = rd * xw[(long long)q * n + d];
+    }
+    const int nblk = gridDim.x;
+    double *pb = partials + (long long)b * nblk * 8;
+    block_sum_store<8>(acc, pb + (long long)blockIdx.x * 8);
+    __shared__ int last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(counters + b, 1u) == unsigned(nblk - 1);
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const int ncomp = 1 + nx + np;
+    if (threadIdx.x < ncomp) {
+        double s = 0.0;
+        for (int k = 0; k < nblk; ++k) s += pb[(long long)k * 8 + threadIdx.x];
+        if (threadIdx.x > nx) s /= n;
+        out[(long long)b * out_bstride + threadIdx.x] = s;
+        if (threadIdx.x == 0 && dscale) {
+            const double re = s * factor, im = 0.0;
+            const double dd = re * re + im * im;
+            dscale[2 * b] = re / dd;
+            dscale[2 * b + 1] = -im / dd;
+        }
+    }
+    if (threadIdx.x == 0) atomicExch(counters + b, 0u);
+}
+
+int launch_cyclic_obs(int prec, const void *data, int n, int batch, double *work, const double *xw, int nx,
+                      const double *pw_hat, int np, double *partials, unsigned *counters, double *out,
+                      long long out_bstride, double factor, double *dscale, hipStream_t st) {
+    const dim3 grid((n + 255) / 256, n / kDiagRowsPerBlock, batch);
+    if (prec == AW_PREC_F64)
+        hipLaunchKernelGGL(k_cyclic_diag<double>, grid, dim3(256), 0, st, data, n, work);
+    else
+        hipLaunchKernelGGL(k_cyclic_diag<float>, grid, dim3(256), 0, st, data, n, work);
+    const dim3 g2((n + 255) / 256, batch);
+    const double2 *ph = reinterpret_cast<const double2 *>(pw_hat);
+    if (prec == AW_PREC_F64)
+        hipLaunchKernelGGL(k_cyclic_obs<double>, g2, dim3(256), 0, st, data, n, work, n / kDiagRowsPerBlock, xw, nx,
+                           ph, np, partials, counters, out, out_bstride, factor, dscale);
+    else
+        hipLaunchKernelGGL(k_cyclic_obs<float>, g2, dim3(256), 0, st, data, n, work, n / kDiagRowsPerBlock, xw, nx,
+                           ph, np, partials, counters, out, out_bstride, factor, dscale);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_cyclic_obs: %s", hipGetErrorString(e));
+    return AW_OK;
+}
+
 int launch_sum_partials(const double *partials, int nblk, int ncomp, int batch, double *out,
                         hipStream_t st) {
     hipLaunchKernelGGL(k_sum_partials, dim3(batch), dim3(64), 0, st, partials, nblk, ncomp, out);

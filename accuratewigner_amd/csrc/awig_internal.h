@@ -58,6 +58,9 @@ constexpr int kWsumMaxBlocks = 512;
 int launch_weighted_sums(int prec, const void *data, long long n_item, const double *w, int nw, int batch,
                          double *partials, unsigned *counters, int nblk, double *out, double factor,
                          double *dscale, hipStream_t st);
+int launch_cyclic_obs(int prec, const void *data, int n, int batch, double *work, const double *xw, int nx,
+                      const double *pw_hat, int np, double *partials, unsigned *counters, double *out,
+                      long long out_bstride, double factor, double *dscale, hipStream_t st);
 int launch_sum_partials(const double *partials, int nblk, int ncomp, int batch, double *out,
                         hipStream_t st);
 int launch_scale_from_stats(const double *stats, int stride, int comp, int mode, double factor,

@@ -39,13 +39,20 @@ template <typename R, int N, bool STRIDED> struct PassTune {
     static constexpr int LPB = fit(WANT, TWM);
 };
 
-// Small passes (lines up to 2048 points, at most kSmallLines lines in the whole batch: one
-// 256^2 state, a few 128^2 states, one 1024^2 state) are bound by the latency of one block, not by HBM: fewer
-// elements per thread (E = N/128, 4..16) and 1-2 lines per block spread the lines over
+// Small passes (lines up to 2048 points and few lines in the whole batch: one 256^2 state,
+// a few 128^2 states, one 1024^2 state) are bound by the latency of one block, not by HBM:
+// fewer elements per thread (E = N/128, 4..16) and 1-2 lines per block spread the lines over
 // many more CUs and shorten each thread's chain.  Twiddles come from LDS.  Measured on
 // MI355X: one 256^2 FP64 state 6.8 -> 4.7 us per pass (the back-to-back launch floor is
-// 2.5 us); no gain once the batch has more than ~2k lines.
-constexpr long long kSmallLines = 2048;
+// 2.5 us).  The crossover to the batch configuration (back-to-back passes, us):
+//   1024 lines: small wins or ties everywhere (1024^2 FP32 row 7.5 vs 8.6, column 10.4 vs 11.3);
+//   2048 lines of 256 points: small still wins (FP32 row 4.8 vs 6.7, FP64 column 7.8 vs 7.9);
+//   2048 lines of 512 points: FP64 loses (column 13.2 vs 10.8), FP32 ties (column 9.9 vs 10.0);
+//   2048 lines of 1024 points: small loses (FP64 row 24.2 vs 16.1, FP32 column 16.5 vs 12.6).
+constexpr long long kSmallLines = 1024;
+template <typename R, int N> constexpr long long small_lines_max() {
+    return (N <= 256 || (sizeof(R) == 4 && N <= 512)) ? 2 * kSmallLines : kSmallLines;
+}
 
 template <typename R, int N, bool STRIDED> struct SmallTune {
     static constexpr int E = (N / 128 < 4) ? 4 : ((N / 128 > 16) ? 16 : N / 128);
@@ -78,7 +85,7 @@ int launch_cfg(const PassArgs &a, hipStream_t st) {
 template <typename R, int N, bool STRIDED>
 int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
     if constexpr (N <= 2048) {
-        if (a.total_lines <= kSmallLines) {
+        if (a.total_lines <= small_lines_max<R, N>()) {
             switch (kind) {
             case 1: return launch_cfg<PassKS<R, N, STRIDED, 0, 1, 0>>(a, st);
             case 2: return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0>>(a, st);

@@ -163,6 +163,16 @@ class Plan:
                   ctypes.c_void_p(current_stream()))
         return out
 
+    def cyclic_diag_obs(self, data, xw, nx, pw_hat, np_, out, out_bstride, factor=0.0, dscale=None):
+        """out[b, :1 + nx + np_] = trace, coordinate sums over diag rho (FP64 weights xw
+        [nx, n]) and momentum sums Re sum_d c_d pw_hat[q, d] / n from the signed cyclic
+        diagonal sums (pw_hat: complex128 [np_, n], DFTs of the momentum weights); with
+        dscale, also dscale = 1 / (trace factor)."""
+        _lib.call("aw_cyclic_diag_obs", self.handle, ctypes.c_void_p(_p(data)), ctypes.c_void_p(_p(xw)), int(nx),
+                  ctypes.c_void_p(_p(pw_hat)), int(np_), ctypes.c_void_p(_p(out)), int(out_bstride),
+                  float(factor), ctypes.c_void_p(_p(dscale)), ctypes.c_void_p(current_stream()))
+        return out
+
     def cyclic_diag_sums(self, data, out):
         """out[batch, n] complex128 (device): (-1)^d sum_i data[i, (i + d) mod n]."""
         _lib.call("aw_cyclic_diag_sums", self.handle, ctypes.c_void_p(_p(data)), ctypes.c_void_p(_p(out)),

@@ -300,27 +300,22 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
 
     def _ehrenfest_chunk(self, n):
         """n steps [EV, I0] [F1, EK, I1] [F0, EV] from the chunk buffers (applying the pending
-        normalisation), each followed by the trace and x-observables from diag rho, the
-        p-observables from the signed cyclic diagonal sums (one read of rho for all three:
-        sum_j [S I1 (g/N) F1 S rho]_jj = Re sum_k g(P_k) F(c)_k / N) and the next scale
-        1 / (Re tr(rho) dX).  Only launches, from fixed addresses."""
+        normalisation), each followed by one native call for the trace, the x-observables
+        from diag rho, the p-observables from the signed cyclic diagonal sums c_d (one read of
+        rho for all: sum_j [S I1 (g/N) F1 S rho]_jj = Re sum_k g(P_k) F(c)_k / N
+        = Re sum_d c_d F(g)_d / N, with the DFTs of the weights F(g) made once per chunk)
+        and the next scale 1 / (Re tr(rho) dX).  Only launches, from fixed addresses."""
         bufs, p, rho = self._ehr, self.plan, self.rho
         N = self.X_gridDIM
         csz = np.dtype(self.state_dtype).itemsize
         colL, colE, rowL, rowE = bufs["fac"]
-        stage, xw, pw = bufs["stage"], bufs["xw"], bufs["pw"]
-        pmod = self._obs_module(["X"])
+        stage, xw, pwc = bufs["stage"], bufs["xw"], bufs["pwc"]
         for k in range(n):
             p.steps(rho, 1, colL.ptr + 2 * k * N * csz, colE.ptr + 2 * k * N * csz,
                     rowL.ptr + k * N * csz, rowE.ptr + k * N * csz, dscale=self._dscale)
             self._scale_pending = False
-            # trace and x-observables, and the next scale 1 / (Re tr(rho) dX)
-            self._diag_obs_ptr(xw.ptr + k * 3 * N * 8, 3, stage.ptr + k * 64, self.dX, self._dscale)
-            p.cyclic_diag_sums(rho, bufs["csum"])
-            self._cplan.fft(bufs["csum"], 1, FWD, out=bufs["chat"])
-            pmod.launch("aw_p_obs", (1,), (256,), [bufs["chat"].ptr, pw.ptr + k * 3 * N * 8, ctypes.c_int(3),
-                                                   ctypes.c_int(N), stage.ptr + (k * 8 + 4) * 8],
-                        current_stream())
+            p.cyclic_diag_obs(rho, xw.ptr + k * 3 * N * 8, 3, pwc.ptr + k * 3 * N * 16, 3, stage.ptr + k * 64, 8,
+                              self.dX, self._dscale)
             self._scale_pending = True
 
     def single_step_propagation(self):
@@ -351,20 +346,19 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
         exprs_p = [self.diff_K, "P", self.K]
         if self._ehr is None:
             p = self.plan
-            self._cplan = Plan(1, N, 1, "fp64")
+            # DFTs of the chunk's momentum weights: CH x 3 rows of N points, FP64
+            self._cplan = Plan(1, N, CH * len(exprs_p), "fp64")
             self._ehr = dict(ts=empty((CH,), np.float64), Vt=empty((CH, N), np.float64),
                              Kt=empty((CH, N), np.float64),
                              fac=(p.new_factors(N, 2 * CH), p.new_factors(N, 2 * CH),
                                   p.new_factors(N, CH), p.new_factors(N, CH)),
                              xw=empty((CH, len(exprs_x), N), np.float64),
-                             pw=empty((CH, len(exprs_p), N), np.float64),
-                             csum=empty((N,), np.complex128), chat=empty((N,), np.complex128),
+                             pwc=DeviceArray((CH, len(exprs_p), N), np.complex128),
                              stage=empty((CH, 8), np.float64), graphs={})
             self._obs_module(exprs_x)
             self._obs_module(exprs_p)
-            self._obs_module(["X"])
-            # the cyclic-diagonal scratch is allocated by its first call, never in a capture
-            self.plan.cyclic_diag_sums(self.rho, self._ehr["csum"])
+            # the observable scratch is allocated by the first call, never in a capture
+            self.plan.cyclic_diag_obs(self.rho, self._ehr["xw"], 3, self._ehr["pwc"], 3, self._ehr["stage"], 8)
         bufs = self._ehr
         if not self._scale_pending:
             # every step applies the pending scale: start from 1 so that all chunks are alike
@@ -377,7 +371,8 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
             self._chunk_factors(tsc, merged=False, bufs=bufs)
             # the observable weights of all the chunk's times at once
             self._weights_t(exprs_x, bufs["ts"], n, bufs["xw"])
-            self._weights_t(exprs_p, bufs["ts"], n, bufs["pw"])
+            self._weights_t(exprs_p, bufs["ts"], n, bufs["pwc"], complex_out=True)
+            self._cplan.fft(bufs["pwc"], 1, FWD)
             graphs.run_chunk(bufs["graphs"], "ehrenfest", lambda n=n: self._ehrenfest_chunk(n), n == CH)
             ring.tensor[c0:c0 + n].copy_(bufs["stage"].tensor[:n])
         self.t = ts[-1]
@@ -399,6 +394,8 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
         mod = self._obs_cache.get(key)
         if mod is None:
             body = "".join("        out[%d * n + k] = (%s);\n" % (q, e) for q, e in enumerate(exprs))
+            cbody = "".join("        outc[%d * n + k] = make_double2((%s), 0.);\n" % (q, e)
+                            for q, e in enumerate(exprs))
             src = (rtc.PRELUDE + self.cuda_consts + "\n#define NOBS_W %d\n" % len(exprs)
                    + "\n__device__ double K(double P, double t)\n{\n    return (%s);\n}\n" % self.K
                    + "\n__device__ double V(double X, double t)\n{\n    return (%s);\n}\n" % self.V
@@ -419,60 +416,17 @@ extern "C" __global__ void aw_weights_t(const double *Xg, const double *Pg, int 
         double *out = out_all + (long long)s * NOBS_W * n;
 %s    }
 }
-extern "C" __global__ void aw_diag_obs(const %s *rho, const double *w, int nobs, int n, double *out,
-                                       double factor, double *dscale)
+extern "C" __global__ void aw_weights_tc(const double *Xg, const double *Pg, int n, const double *ts, int nt,
+                                         double2 *outc_all)
 {
-    // out[0] = Re tr rho, out[1 + q] = sum_j Re rho_jj w_q[j]  (one block); with dscale, also
-    // the normalisation scale 1 / (out[0] factor) (the formula of libawig's k_scale_from_stats)
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        const double d = (double)rho[(long long)k * n + k].x;
-        acc[0] += d;
-        for (int q = 0; q < nobs && q < 7; ++q) acc[1 + q] += d * w[q * n + k];
-    }
-    __shared__ double red[4][8];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int c = 0; c < 8; ++c) {
-        const double s = aw_wave_sum(acc[c]);
-        if (lane == 0) red[wid][c] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < 8) {
-        double s = 0.;
-        for (int q = 0; q < 4; ++q) s += red[q][threadIdx.x];
-        if (threadIdx.x <= nobs) out[threadIdx.x] = s;
-        if (threadIdx.x == 0 && dscale) {
-            const double re = s * factor, im = 0.;
-            const double d = re * re + im * im;
-            dscale[0] = re / d;
-            dscale[1] = -im / d;
-        }
-    }
+    // complex (imaginary part 0) copy of aw_weights_t's output, ready for a DFT
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n * nt; idx += gridDim.x * blockDim.x) {
+        const int s = idx / n, k = idx - s * n;
+        const double X = Xg[k], P = Pg[k], t = ts[s];
+        double2 *outc = outc_all + (long long)s * NOBS_W * n;
+%s    }
 }
-extern "C" __global__ void aw_copy_scalar(const double *src, double *dst) { *dst = *src; }
-extern "C" __global__ void aw_p_obs(const double2 *chat, const double *w, int nobs, int n, double *out)
-{
-    // out[q] = sum_k w_q[k] Re chat[k] / n (one block): momentum observables from the
-    // DFT of the signed cyclic diagonal sums of rho
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        const double c = chat[k].x;
-        for (int q = 0; q < nobs && q < 8; ++q) acc[q] += c * w[q * n + k];
-    }
-    __shared__ double red[4][8];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int c = 0; c < 8; ++c) {
-        const double s = aw_wave_sum(acc[c]);
-        if (lane == 0) red[wid][c] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < 8 && threadIdx.x < nobs) {
-        double s = 0.;
-        for (int q = 0; q < 4; ++q) s += red[q][threadIdx.x];
-        out[threadIdx.x] = s / n;
-    }
-}
-""" % (body, body, self._state_t()))
+""" % (body, body, cbody))
             mod = rtc.Module(src, "rho_vneumann_obs.hip")
             self._obs_cache[key] = mod
         return mod
@@ -487,30 +441,15 @@ extern "C" __global__ void aw_p_obs(const double2 *chat, const double *w, int no
                    [self._dev_kX.ptr, self._dev_kP.ptr, ctypes.c_int(N), ctypes.c_double(t), out.ptr],
                    current_stream())
 
-    def _diag_obs(self, w, nobs, out_ptr):
-        self._diag_obs_ptr(w.ptr, nobs, out_ptr)
-
-    def _diag_obs_ptr(self, w_ptr, nobs, out_ptr, factor=0.0, dscale=None):
-        """out[0] = Re tr rho, out[1 + q] = sum_j Re rho_jj w_q[j]; with dscale, also the
-        normalisation scale 1 / (out[0] factor)."""
-        mod = self._obs_module(["X"])
-        mod.launch("aw_diag_obs", (1,), (256,), [self.rho.ptr, w_ptr, ctypes.c_int(nobs),
-                                                  ctypes.c_int(self.X_gridDIM), out_ptr, ctypes.c_double(factor),
-                                                  ctypes.c_void_p(dscale.ptr if dscale is not None else None)],
-                   current_stream())
-
-    def _weights_t(self, exprs, ts_dev, nt, out):
-        """out[s][q][k] = expr_q(X_k, P_k, ts[s]) for s < nt (one launch per chunk)."""
+    def _weights_t(self, exprs, ts_dev, nt, out, complex_out=False):
+        """out[s][q][k] = expr_q(X_k, P_k, ts[s]) for s < nt (one launch per chunk); as
+        complex128 (imaginary part 0) with complex_out."""
         N = self.X_gridDIM
         mod = self._obs_module(exprs)
         blocks, threads = rtc.launch_cfg(N * nt)
-        mod.launch("aw_weights_t", (blocks,), (threads,),
+        mod.launch("aw_weights_tc" if complex_out else "aw_weights_t", (blocks,), (threads,),
                    [self._dev_kX.ptr, self._dev_kP.ptr, ctypes.c_int(N), ts_dev.ptr, ctypes.c_int(nt), out.ptr],
                    current_stream())
-
-    def _copy_scalar(self, stats, comp, dst_ptr):
-        mod = self._obs_module(["X"])
-        mod.launch("aw_copy_scalar", (1,), (1,), [stats.ptr + comp * 8, dst_ptr], current_stream())
 
     def _weight_vec(self, w, q, N):
         """Complex factor vector (plan precision) holding the real weights w[q]."""

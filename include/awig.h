@@ -194,6 +194,23 @@ int aw_weighted_sums(aw_plan *plan, const void *data, const double *weights, int
                      double factor, double *dscale, aw_stream stream);
 
 /*
+ * Per-step Ehrenfest sums of square density matrices (n >= 64), FP64, two launches:
+ *   out[b][0]          = Re tr rho[b],
+ *   out[b][1 + q]      = sum_i Re rho[b][i][i] xw[q][i]                  (q < nx),
+ *   out[b][1 + nx + q] = Re sum_d c_d pw_hat[q][d] / n                   (q < np),
+ * with c_d = (-1)^d sum_i rho[b][i][(i + d) mod n] and pw_hat[q] the forward DFT of a real
+ * momentum weight pw[q] (complex128 [np][n]), so that the last group equals
+ * sum_k pw[q][k] Re FFT(c)_k / n: the averages that the reference's get_Ehrenfest
+ * (rho_vneumann_cuda_1d.py:470-501) takes through get_average (:520-563), i.e. weighted
+ * traces after the ket index is moved to momentum space.  xw: FP64 [nx][n].  1 + nx + np <= 8; rows of out are
+ * out_bstride doubles apart.  With dscale != NULL also dscale[b] = 1 / (out[b][0] * factor).
+ * The first call on a plan allocates its scratch and must not be made under stream capture.
+ */
+int aw_cyclic_diag_obs(aw_plan *plan, const void *data, const double *xw, int nx, const double *pw_hat,
+                       int np, double *out, long long out_bstride, double factor, double *dscale,
+                       aw_stream stream);
+
+/*
  * Signed cyclic diagonal sums of square items (n >= 64), FP64 complex:
  *   out[b][d] = (-1)^d sum_i data[b][i][(i + d) mod n],  d = 0..n-1   (2 n doubles per item).
  * The forward DFT of this vector is the momentum-space diagonal of rho scaled by n, so a

@@ -31,8 +31,9 @@ def rel(a, b):
 @pytest.mark.parametrize("n", [64, 128, 256, 512, 1024, 2048])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_fft_axis_matches_numpy(n, prec, big):
-    """Both kernel configurations: small batches (at most 2048 lines per pass, latency
-    configuration) and batches above that (throughput configuration)."""
+    """Both kernel configurations: small batches (few lines per pass, latency configuration:
+    up to 1024 lines, or 2048 of at most 256 points / 512 in FP32) and batches above that
+    (throughput configuration)."""
     import torch
     from accuratewigner_amd.engine import Plan, FWD, INV
     from accuratewigner_amd.device import to_device
