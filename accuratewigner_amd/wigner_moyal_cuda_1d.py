@@ -639,22 +639,25 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             self._observable_kernel(exprs)
             self._obs_partials_for(nobs)             # allocated here, never inside a capture
             g = self._obs_weights(exprs) if self.obs_by_adjoint else None
-            self._ehr["weights"] = g
+            # time-independent observables imply a time-independent V (the Hamiltonian is one of
+            # them): the steps are merged and the observables read from each row-pass output
+            self._ehr["weights"] = self._closing_adjoint(g) if g is not None else None
             if g is not None:
                 # the first call allocates the plan's scratch (never inside a capture)
-                self.plan.weighted_sums(self.rho, g, nobs, self._ehr["stage"])
+                self.plan.weighted_sums(self.rho, self._ehr["weights"], nobs, self._ehr["stage"])
         bufs = self._ehr
         if not self._scale_pending:
             # every step applies the pending scale: start from 1 so that all chunks are alike
             self._dscale.tensor[:, 0] = 1.0
             self._dscale.tensor[:, 1] = 0.0
             self._scale_pending = True
+        merged = bufs["weights"] is not None
+        chunk = self._ehrenfest_merged_chunk if merged else self._ehrenfest_chunk
         for c0 in range(0, steps, CH):
             tsc = ts[c0:c0 + CH]
             n = len(tsc)
-            self._build_chunk_factors(tsc, merged=False, bufs=bufs)
-            graphs.run_chunk(bufs["graphs"], tuple(exprs), lambda n=n: self._ehrenfest_chunk(exprs, n),
-                             n == CH)
+            self._build_chunk_factors(tsc, merged=merged, bufs=bufs)
+            graphs.run_chunk(bufs["graphs"], (merged,) + tuple(exprs), lambda n=n: chunk(exprs, n), n == CH)
             ring.tensor[c0:c0 + n].copy_(bufs["stage"].tensor[:n])
         self.t = ts[-1]
         # final normalisation of rho and W by the last step's norm
@@ -694,23 +697,68 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         nobs = len(exprs)
         row_bytes = self.batch * nobs * 8
         stage, ts = bufs["stage"], bufs["ts"]
-        g = bufs["weights"]
         for k in range(n):
             # one step = three passes; its opening / closing vectors sit at 2 k and 2 k + 1
             p.steps(rho, 1, colL.ptr + 2 * k * Nx * csz, colE.ptr + 2 * k * Np * csz,
                     rowL.ptr + k * Np * csz, rowE.ptr + k * Nx * csz, mask_open_close=tab1,
                     mask_row=tabp, dscale=self._dscale)
             self._scale_pending = False
-            if g is not None:
-                # time-independent observables: Re <g_k, rho> and the next scale, one launch
-                p.weighted_sums(rho, g, nobs, stage.ptr + k * row_bytes, self.dXdP, self._dscale)
-                self._scale_pending = True
-                continue
             self.rho2wigner_()
             # observables into the stage row and the next scale 1 / (norm dXdP), one launch
             self._obs_fin_launch(exprs, self.wignerfunction, ts.ptr + 8 * k, stage.ptr + k * row_bytes,
                                  bufs["counters"], self.dXdP, self._dscale)
             self._scale_pending = True
+
+    def _ehrenfest_merged_chunk(self, exprs, n):
+        """n steps with time-independent observables (and so a time-independent V), merged
+        like the plain loop: [EV_0 -> I0] ([F1 -> EK_s -> I1] [sums] [F0 -> EV EV -> I0])*
+        ending [F0 -> EV].  The step-complete rho_s = C1 F0(Y_s), with Y_s the row-pass output
+        and C1 the closing half-kick, so its observables Re <g, rho_s> = Re <g', Y_s> with
+        g' = I0(conj(C1) g) (``_closing_adjoint``); the sums also give the normalisation,
+        which the next merged column pass applies.  Two passes and one reduction per step.
+        Only launches, from fixed addresses."""
+        bufs, p, rho = self._ehr, self.plan, self.rho
+        colL, colE, rowL, rowE = bufs["fac"]
+        _, _, tab1 = self._mask_args(0, 1)
+        _, _, tab2 = self._mask_args(0, 2)
+        _, _, tabp = self._mask_args(1, 1)
+        csz = np.dtype(self.state_dtype).itemsize
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        nobs = len(exprs)
+        row_bytes = self.batch * nobs * 8
+        stage, gy = bufs["stage"], bufs["weights"]
+        p.run(rho, 0, 0, INV, line_fac=colL, elem_fac=colE, mask=tab1, dscale=self._dscale)
+        for k in range(n):
+            p.run(rho, 1, FWD, INV, line_fac=rowL.ptr + k * Np * csz, elem_fac=rowE.ptr + k * Nx * csz,
+                  mask=tabp)
+            p.weighted_sums(rho, gy, nobs, stage.ptr + k * row_bytes, self.dXdP, self._dscale)
+            last = k == n - 1
+            p.run(rho, 0, FWD, 0 if last else INV, line_fac=colL.ptr + (k + 1) * Nx * csz,
+                  elem_fac=colE.ptr + (k + 1) * Np * csz, mask=tab1 if last else tab2,
+                  dscale=None if last else self._dscale)
+        self._scale_pending = True
+
+    def _closing_adjoint(self, g):
+        """g' = I0(conj(C1) g) for each weight array g_q (FP64): C1 is the closing half-kick of
+        a step, (-1)^(i+j) exp(-i dt/2 [V(XX_j) - V(XX'_i)]) / Np times the coordinate mask,
+        as the merged sequence's last column vector builds it.  V must not depend on t."""
+        Nx, Np, dt = self.X_gridDIM, self.P_gridDIM, self.dt
+        p64 = self._obs_plan
+        ts = to_device(np.array([float(self.t)]))
+        Vj, Vi = empty((1, Nx), np.float64), empty((1, Np), np.float64)
+        self._eval_1d(0, self._dev_kXX, Nx, ts, 1, 0.5 * dt, Vj)
+        self._eval_1d(0, self._dev_kXXp, Np, ts, 1, 0.5 * dt, Vi)
+        mxp, mx, _ = self._mask_args(0, 1)
+        cl, ce = p64.new_factors(Nx, 1), p64.new_factors(Np, 1)
+        # conjugates of the closing vectors: the phase coefficients change sign
+        p64.factors(cl, Nx, 1, A=Vj.ptr, ca=0.5 * dt, mag=mx.ptr if mx is not None else None, alt_sign=True)
+        p64.factors(ce, Np, 1, A=Vi.ptr, ca=-0.5 * dt, mag=mxp.ptr if mxp is not None else None,
+                    alt_sign=True, scale=1.0 / Np)
+        m = self._masks.get(0)
+        table = to_device(self._eval_mask(0, float(self.t))) if m is not None and m[0] == "table" else None
+        gy = g.copy()
+        p64.run(gy, 0, 0, INV, line_fac=cl, elem_fac=ce, mask=table)
+        return gy
 
     def get_Ehrenfest(self, t):
         """wigner_moyal_cuda_1d.py:584-606 (normalises, then appends the five averages)."""

@@ -1,6 +1,7 @@
-"""Ehrenfest averages of time-independent observables recorded as Re <L^H f, rho> (one
-weighted sum per step, L = the Wigner transform) against the same run that forms W every
-step and sums Re(W) f; time-dependent observables keep the per-step transform."""
+"""Ehrenfest averages of time-independent observables recorded as Re <g', Y> on each
+row-pass output Y of merged steps (g' = I0(conj(C1) L^H f), L = the Wigner transform, C1 the
+closing half-kick) against the same run with unmerged steps that forms W every step and
+sums Re(W) f; time-dependent observables keep the per-step transform."""
 import numpy as np
 import pytest
 
@@ -25,7 +26,9 @@ def _run(by_adjoint, precision="fp64", batch=1, steps=300, V="0.5 * X * X + 0.1 
     return s
 
 
-@pytest.mark.parametrize("mask", [None, "exp(-pow(XX / 10., 20)) * exp(-pow(XX_prime / 10., 20))"])
+# no mask, a separable mask (folded into the factor vectors) and a non-separable one (a table)
+@pytest.mark.parametrize("mask", [None, "exp(-pow(XX / 10., 20)) * exp(-pow(XX_prime / 10., 20))",
+                                  "exp(-pow((XX + 0.5 * XX_prime) / 12., 20))"])
 def test_adjoint_weights_match_the_per_step_transform_fp64(mask):
     a, b = _run(True, mask=mask), _run(False, mask=mask)
     assert a._ehr["weights"] is not None and b._ehr["weights"] is None
