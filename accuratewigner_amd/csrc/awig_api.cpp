@@ -353,17 +353,19 @@ static int ensure_finish_scratch(aw_plan *p, aw_stream stream, const char *who) 
     return AW_OK;
 }
 
-int aw_weighted_sums(aw_plan *p, const void *data, const double *weights, int nw, double *out, double factor,
-                     double *dscale, aw_stream stream) {
+int aw_weighted_sums(aw_plan *p, const void *data, const double *weights, int nw, int norm2, double *out,
+                     double factor, double *dscale, aw_stream stream) {
     if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
     if (!data || !weights || !out) return set_error(AW_ERR_INVALID_VALUE, "NULL data, weights or out");
-    if (nw < 1 || nw > 8) return set_error(AW_ERR_INVALID_VALUE, "nw must be 1..8 (got %d)", nw);
+    if (nw < 1 || nw + (norm2 ? 1 : 0) > 8)
+        return set_error(AW_ERR_INVALID_VALUE, "nw must be 1..%d (got %d)", norm2 ? 7 : 8, nw);
     const int rc = ensure_finish_scratch(p, stream, "aw_weighted_sums");
     if (rc != AW_OK) return rc;
     const long long n_item = (long long)p->rows * p->cols;
+    // 8 elements per thread (4 independent loads in flight), at most 512 blocks
     const long long nb = n_item / 2048;
     const int nblk = int(nb < 1 ? 1 : (nb > kWsumMaxBlocks ? kWsumMaxBlocks : nb));
-    return launch_weighted_sums(p->prec, data, n_item, weights, nw, p->batch, p->wsum_partials, p->wsum_counters,
+    return launch_weighted_sums(p->prec, data, n_item, weights, nw, norm2 ? 1 : 0, p->batch, p->wsum_partials,
                                 nblk, out, factor, dscale, (hipStream_t)stream);
 }
 
@@ -390,6 +392,13 @@ int aw_cyclic_diag_obs(aw_plan *p, const void *data, const double *xw, int nx, c
     if (rc != AW_OK) return rc;
     return launch_cyclic_obs(p->prec, data, n, p->batch, p->diag_work, xw, nx, pw_hat, np, p->wsum_partials,
                              p->wsum_counters, out, out_bstride, factor, dscale, (hipStream_t)stream);
+}
+
+int aw_finish_sums(const double *partials, int nblk, int ncomp, int batch, double *out, double factor,
+                   double *dscale, aw_stream stream) {
+    if (!partials || !out || nblk < 1 || ncomp < 1 || ncomp > 8 || batch < 1)
+        return set_error(AW_ERR_INVALID_VALUE, "bad partial-sum arguments");
+    return launch_finish_sums(partials, nblk, ncomp, batch, out, factor, dscale, (hipStream_t)stream);
 }
 
 int aw_sum_partials(const double *partials, int nblk, int ncomp, int batch, double *out,

@@ -204,25 +204,78 @@ int launch_pair_dot(int prec, const void *a, const void *b, long long n_item, in
     return AW_OK;
 }
 
-// Real parts of weighted sums of each item: out[b][q] = Re sum_i conj(w[q][i]) z[b][i] for
-// q < NW, FP64 weights shared by the items.  Block partials are added by the last block of
-// each item to finish (ticket counter), in block order, so one launch gives deterministic
-// sums; that block also writes the normalisation scale 1 / (out[b][0] factor) when dscale is
-// given and resets the item's counter for the next launch (replayable from a graph).
-template <typename R, int NW>
+// Second stage of the block-partial reductions: one block per item adds the nblk partials of
+// each of the ncomp components (thread t takes k = t, t + 256, ... in order, then a fixed-order
+// block sum), writes them to out[b][c] and, when dscale is given, the normalisation scale
+// 1 / (out[b][0] factor) (the formula of k_scale_from_stats, mode 0).  Deterministic.
+// A one-launch "last block finishes" protocol was measured instead: its device-scope fence
+// and ticket cost about 0.16 us per block, serialised (128 blocks: 25 us against 3 + 3 us).
+__global__ __launch_bounds__(RED_THREADS) void k_finish_sums(const double *partials, int nblk, int ncomp,
+                                                             double *out, double factor, double *dscale) {
+    const int b = blockIdx.x;
+    const double *pb = partials + (long long)b * nblk * ncomp;
+    double tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = threadIdx.x; k < nblk; k += RED_THREADS) {
+        for (int c = 0; c < ncomp && c < 8; ++c) tot[c] += pb[(long long)k * ncomp + c];
+    }
+    __shared__ double fin[8];
+    block_sum_store<8>(tot, fin);
+    __syncthreads();
+    if (threadIdx.x < ncomp) {
+        const double sum = fin[threadIdx.x];
+        out[(long long)b * ncomp + threadIdx.x] = sum;
+        if (threadIdx.x == 0 && dscale) {
+            const double re = sum * factor, im = 0.0;
+            const double d = re * re + im * im;
+            dscale[2 * b] = re / d;
+            dscale[2 * b + 1] = -im / d;
+        }
+    }
+}
+
+int launch_finish_sums(const double *partials, int nblk, int ncomp, int batch, double *out, double factor,
+                       double *dscale, hipStream_t st) {
+    hipLaunchKernelGGL(k_finish_sums, dim3(batch), dim3(RED_THREADS), 0, st, partials, nblk, ncomp, out, factor,
+                       dscale);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_finish_sums: %s", hipGetErrorString(e));
+    return AW_OK;
+}
+
+// Real parts of weighted sums of each item: partials of out[b][q] = Re sum_i conj(w[q][i]) z[b][i]
+// for q < NW, FP64 weights shared by the items, and with N2 also of sum_i |z[b][i]|^2.
+template <typename R, int NW, int N2>
 __global__ __launch_bounds__(RED_THREADS) void k_weighted_sums(const void *data, long long n_item,
-                                                                 const double2 *w, double *partials,
-                                                                 unsigned *counters, double *out,
-                                                                 double factor, double *dscale) {
+                                                                 const double2 *w, double *partials) {
     using T2 = typename Cx<R>::T;
+    constexpr int NC = NW + N2;
+    constexpr int U = 4;   // independent element loads in flight per thread
     const int b = blockIdx.y;
     const int nblk = gridDim.x;
     const T2 *z = reinterpret_cast<const T2 *>(data) + (long long)b * n_item;
-    double acc[NW];
+    double acc[NC];
 #pragma unroll
-    for (int q = 0; q < NW; ++q) acc[q] = 0.0;
+    for (int q = 0; q < NC; ++q) acc[q] = 0.0;
     const long long stride = (long long)nblk * RED_THREADS;
-    for (long long i = (long long)blockIdx.x * RED_THREADS + threadIdx.x; i < n_item; i += stride) {
+    long long i = (long long)blockIdx.x * RED_THREADS + threadIdx.x;
+    for (; i + (U - 1) * stride < n_item; i += U * stride) {
+        T2 v[U];
+        double2 c[U][NW];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            v[u] = z[i + u * stride];
+#pragma unroll
+            for (int q = 0; q < NW; ++q) c[u][q] = w[q * n_item + i + u * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double x = v[u].x, y = v[u].y;
+#pragma unroll
+            for (int q = 0; q < NW; ++q) acc[q] += c[u][q].x * x + c[u][q].y * y;
+            if constexpr (N2 != 0) acc[NW] += x * x + y * y;
+        }
+    }
+    for (; i < n_item; i += stride) {
         const T2 v = z[i];
         const double x = v.x, y = v.y;
 #pragma unroll
@@ -230,45 +283,30 @@ __global__ __launch_bounds__(RED_THREADS) void k_weighted_sums(const void *data,
             const double2 c = w[q * n_item + i];
             acc[q] += c.x * x + c.y * y;
         }
+        if constexpr (N2 != 0) acc[NW] += x * x + y * y;
     }
-    double *pb = partials + (long long)b * nblk * NW;
-    block_sum_store<NW>(acc, pb + (long long)blockIdx.x * NW);
-    // the partials of this block are stored; the last block of the item adds them all
-    __shared__ int last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(counters + b, 1u) == unsigned(nblk - 1);
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    if (threadIdx.x < NW) {
-        double s = 0.0;
-        for (int k = 0; k < nblk; ++k) s += pb[(long long)k * NW + threadIdx.x];
-        out[(long long)b * NW + threadIdx.x] = s;
-        if (threadIdx.x == 0 && dscale) {
-            const double re = s * factor, im = 0.0;
-            const double d = re * re + im * im;
-            dscale[2 * b] = re / d;
-            dscale[2 * b + 1] = -im / d;
-        }
-    }
-    if (threadIdx.x == 0) atomicExch(counters + b, 0u);
+    block_sum_store<NC>(acc, partials + ((long long)b * nblk + blockIdx.x) * NC);
 }
 
 template <typename R, int NW>
-static void launch_wsums_t(const void *data, long long n_item, const double *w, int batch, double *partials,
-                           unsigned *counters, int nblk, double *out, double factor, double *dscale,
-                           hipStream_t st) {
-    hipLaunchKernelGGL((k_weighted_sums<R, NW>), dim3(nblk, batch), dim3(RED_THREADS), 0, st, data, n_item,
-                       reinterpret_cast<const double2 *>(w), partials, counters, out, factor, dscale);
+static void launch_wsums_t(const void *data, long long n_item, const double *w, int norm2, int batch,
+                           double *partials, int nblk, hipStream_t st) {
+    if constexpr (NW < 8) {
+        if (norm2) {
+            hipLaunchKernelGGL((k_weighted_sums<R, NW, 1>), dim3(nblk, batch), dim3(RED_THREADS), 0, st, data,
+                               n_item, reinterpret_cast<const double2 *>(w), partials);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_weighted_sums<R, NW, 0>), dim3(nblk, batch), dim3(RED_THREADS), 0, st, data, n_item,
+                       reinterpret_cast<const double2 *>(w), partials);
 }
 
 template <typename R>
-static int launch_wsums_r(const void *data, long long n_item, const double *w, int nw, int batch,
-                          double *partials, unsigned *counters, int nblk, double *out, double factor,
-                          double *dscale, hipStream_t st) {
+static int launch_wsums_r(const void *data, long long n_item, const double *w, int nw, int norm2, int batch,
+                          double *partials, int nblk, hipStream_t st) {
 #define AW_WS(K) \
-    case K: launch_wsums_t<R, K>(data, n_item, w, batch, partials, counters, nblk, out, factor, dscale, st); break;
+    case K: launch_wsums_t<R, K>(data, n_item, w, norm2, batch, partials, nblk, st); break;
     switch (nw) {
         AW_WS(1) AW_WS(2) AW_WS(3) AW_WS(4) AW_WS(5) AW_WS(6) AW_WS(7) AW_WS(8)
     default: return set_error(AW_ERR_INVALID_VALUE, "nw must be 1..8 (got %d)", nw);
@@ -277,18 +315,16 @@ static int launch_wsums_r(const void *data, long long n_item, const double *w, i
     return AW_OK;
 }
 
-int launch_weighted_sums(int prec, const void *data, long long n_item, const double *w, int nw, int batch,
-                         double *partials, unsigned *counters, int nblk, double *out, double factor,
-                         double *dscale, hipStream_t st) {
+int launch_weighted_sums(int prec, const void *data, long long n_item, const double *w, int nw, int norm2,
+                         int batch, double *partials, int nblk, double *out, double factor, double *dscale,
+                         hipStream_t st) {
     const int rc = prec == AW_PREC_F64
-                       ? launch_wsums_r<double>(data, n_item, w, nw, batch, partials, counters, nblk, out,
-                                                factor, dscale, st)
-                       : launch_wsums_r<float>(data, n_item, w, nw, batch, partials, counters, nblk, out,
-                                               factor, dscale, st);
+                       ? launch_wsums_r<double>(data, n_item, w, nw, norm2, batch, partials, nblk, st)
+                       : launch_wsums_r<float>(data, n_item, w, nw, norm2, batch, partials, nblk, st);
     if (rc != AW_OK) return rc;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_weighted_sums: %s", hipGetErrorString(e));
-    return AW_OK;
+    return launch_finish_sums(partials, nblk, nw + norm2, batch, out, factor, dscale, st);
 }
 
 // signed cyclic diagonal sums of square items: out[b][d] = (-1)^d sum_i z[i][(i + d) mod n]

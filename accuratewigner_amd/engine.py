@@ -154,12 +154,13 @@ class Plan:
                   ctypes.c_void_p(current_stream()))
         return out
 
-    def weighted_sums(self, data, weights, nw, out, factor=0.0, dscale=None):
-        """out[batch, nw] (device FP64) = Re sum conj(weights[q]) data per item (weights:
-        complex128 [nw, rows, cols]); with dscale, also dscale = 1 / (out[:, 0] factor).  The
-        first call on a plan allocates its scratch (never under graph capture)."""
+    def weighted_sums(self, data, weights, nw, out, factor=0.0, dscale=None, norm2=False):
+        """out[batch, nw (+1)] (device FP64) = Re sum conj(weights[q]) data per item (weights:
+        complex128 [nw, rows, cols]), and with norm2 also sum |data|^2 last; with dscale, also
+        dscale = 1 / (out[:, 0] factor).  The first call on a plan allocates its scratch
+        (never under graph capture)."""
         _lib.call("aw_weighted_sums", self.handle, ctypes.c_void_p(_p(data)), ctypes.c_void_p(_p(weights)),
-                  int(nw), ctypes.c_void_p(_p(out)), float(factor), ctypes.c_void_p(_p(dscale)),
+                  int(nw), int(bool(norm2)), ctypes.c_void_p(_p(out)), float(factor), ctypes.c_void_p(_p(dscale)),
                   ctypes.c_void_p(current_stream()))
         return out
 
@@ -188,6 +189,13 @@ class Plan:
         n = self.rows * self.cols if n is None else n
         _lib.call("aw_scale", self.prec, ctypes.c_void_p(_p(data)), n, self.batch,
                   ctypes.c_void_p(_p(dscale)), scale, ctypes.c_void_p(current_stream()))
+
+
+def finish_sums(partials, nblk, ncomp, batch, out, factor=0.0, dscale=None):
+    """out[b, c] = sum of the nblk block partials of component c (one block per item, fixed
+    order); with dscale, also dscale[b] = 1 / (out[b, 0] factor)."""
+    _lib.call("aw_finish_sums", ctypes.c_void_p(_p(partials)), int(nblk), int(ncomp), int(batch),
+              ctypes.c_void_p(_p(out)), float(factor), ctypes.c_void_p(_p(dscale)), ctypes.c_void_p(current_stream()))
 
 
 def sum_partials(partials, nblk, ncomp, batch, out):

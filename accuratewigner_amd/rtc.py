@@ -176,41 +176,6 @@ REDUCE_TAIL = r"""
     }
 """
 
-# After REDUCE_TAIL: the last block of item blockIdx.y to finish adds the partials of all of
-# the item's blocks in block order (the order of libawig's k_sum_partials, so the sums are
-# bit-identical to the separate reduction launch), writes them to out[item][c] and derives
-# the next normalisation scale 1 / (out[item][0] * factor) as libawig's k_scale_from_stats
-# does (mode 0).  counters[item] starts at zero and is reset by that last block, so the
-# kernel can be replayed from a captured graph.  Device-scope fences order the partial-sum
-# stores before the ticket and the ticket before the loads.
-REDUCE_FINISH = r"""
-__device__ __forceinline__ void aw_finish(const double *partials, unsigned *counters, double *out,
-                                          double factor, double *dscale)
-{
-    __shared__ int aw_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) aw_last = atomicAdd(counters + blockIdx.y, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!aw_last) return;
-    __threadfence();
-    if (threadIdx.x < %(nobs)d) {
-        const double *pb = partials + (long long)blockIdx.y * gridDim.x * %(nobs)d;
-        double s = 0.;
-        for (int i = 0; i < (int)gridDim.x; ++i) s += pb[(long long)i * %(nobs)d + threadIdx.x];
-        out[(long long)blockIdx.y * %(nobs)d + threadIdx.x] = s;
-        if (threadIdx.x == 0) {
-            const double re = s * factor, im = 0.;
-            const double d = re * re + im * im;
-            dscale[2 * blockIdx.y] = re / d;
-            dscale[2 * blockIdx.y + 1] = -im / d;
-        }
-    }
-    if (threadIdx.x == 0) atomicExch(counters + blockIdx.y, 0u);
-}
-"""
-
-
 def launch_cfg(total, threads=256, max_blocks=65536):
     blocks = max(1, min(max_blocks, (total + threads - 1) // threads))
     return blocks, threads

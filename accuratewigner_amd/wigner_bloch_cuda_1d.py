@@ -10,6 +10,7 @@ step: [EVb, I0] [F1, EKb, I1] [F0, EVb].
 """
 import numpy as np
 
+from . import graphs
 from .device import empty, to_device
 from .engine import FWD, INV
 from .wigner_moyal_cuda_1d import WignerMoyalCUDA1D
@@ -25,6 +26,9 @@ def _py2_round(x):
 class WignerBlochCUDA1D(WignerMoyalCUDA1D):
     """Gibbs state rho = exp(-H/kT) of H = K(p) + V(x) by Bloch propagation in phase space
     (reference: wigner_bloch_cuda_1d.py:9-18)."""
+
+    # get_gibbs_state stops when the purity is not below this (the reference's check, :147)
+    purity_limit = 1.
 
     def __init__(self, **kwargs):
         """Keywords of WignerMoyalCUDA1D plus kT, dbeta and t_initial
@@ -44,6 +48,8 @@ class WignerBlochCUDA1D(WignerMoyalCUDA1D):
         self.cuda_consts += "#define V_min %.15e \n" % self.V_min
         self.cuda_consts += "#define K_min %.15e \n" % self.K_min
         self._bloch_cache = {}
+        self._gibbs_w = {}
+        self._gibbs_bufs = {}      # persistent per-chunk sums and captured chunks of the Gibbs loop
         self.gibbs_stopped_at = None
 
     def _eval_at_t0(self, which, xdev, n):
@@ -137,11 +143,30 @@ class WignerBlochCUDA1D(WignerMoyalCUDA1D):
             num_beta_steps = np.inf
         return dbeta, num_beta_steps
 
-    def _purities(self, stats_host, n):
-        # purity = 2 pi Re sum(W^2) dXdP with W normalised by n = Re sum W dXdP (:349-354)
-        s = stats_host.reshape(n, self.batch, 8)
+    def _purities(self, stats_host, n, ncomp=8, comp=3, scale=1.0):
+        # purity = 2 pi Re sum(W^2) dXdP with W normalised by n = Re sum W dXdP (:349-354);
+        # stats rows hold Re sum W in component 0 and sum W^2 / scale in component comp
+        s = stats_host.reshape(n, self.batch, ncomp)
         norm = s[:, :, 0] * self.dXdP
-        return 2. * np.pi * s[:, :, 3] * self.dXdP / norm ** 2
+        return 2. * np.pi * scale * s[:, :, comp] * self.dXdP / norm ** 2
+
+    def _gibbs_weights(self, record):
+        """Weights of the Gibbs loop's per-step sums, FP64 [1 + m, Np, Nx]: L^H 1 (Re sum W,
+        the norm) and L^H f for the m recorded expressions at the current t, with L the Wigner
+        transform (the sums are linear in rho).  Cached per expression list and t."""
+        key = (tuple(record), float(self.t))
+        g = self._gibbs_w.get(key)
+        if g is None:
+            g = self._gibbs_w[key] = self._obs_weights(["1."] + list(record), t=float(self.t))
+        return g
+
+    def _normalized_step_sums(self, dbeta, g, nw, out_ptr):
+        """One Bloch step, then one launch for Re sum W, the recorded sums and sum |rho|^2
+        (sum W^2 = Np sum |rho|^2: the Wigner transform is Np times an isometry) into out
+        (nw + 1 values per item), and the next normalisation scale.  W is not formed."""
+        self._bloch_step_passes(dbeta)
+        self.plan.weighted_sums(self.rho, g, nw, out_ptr, self.dXdP, self._dscale, norm2=True)
+        self._scale_pending = True
 
     def get_gibbs_state(self, kT=None, dbeta=None, record=None):
         """wigner_bloch_cuda_1d.py:119-155: infinite-temperature start, n Bloch steps with a
@@ -159,6 +184,25 @@ class WignerBlochCUDA1D(WignerMoyalCUDA1D):
         dbeta = np.float64(dbeta)
         self.set_wignerfunction(1. / np.prod(self.wignerfunction.shape[-2:]))
         self.gibbs_stopped_at = None
+        record = list(record) if record else []
+        m = len(record)
+        fast = self.obs_by_adjoint and m + 2 <= 8
+        g = self._gibbs_weights(record) if fast else None
+        nc = m + 2 if fast else 8         # per-step values per item in `stats`
+        if fast:
+            # every step applies the pending scale (1 at the start), so that the chunks are
+            # alike and a full chunk can be captured once and replayed (fixed addresses)
+            if not self._scale_pending:
+                self._dscale.tensor[:, 0] = 1.0
+                self._dscale.tensor[:, 1] = 0.0
+                self._scale_pending = True
+            fb = self._gibbs_bufs.get(nc)
+            if fb is None:
+                fb = self._gibbs_bufs[nc] = dict(stats=empty((CHECK_CHUNK, self.batch, nc), np.float64), graphs={})
+                # the first call allocates the plan's scratch (never inside a capture)
+                self.plan.weighted_sums(self.rho, g, m + 1, fb["stats"], norm2=True)
+            self._bloch_factors(float(dbeta))      # cached factor vectors, made outside any capture
+            gkey = (float(dbeta), tuple(record), float(self.t))
         rec = []
         k = 0
         while k < num_beta_steps:
@@ -166,14 +210,27 @@ class WignerBlochCUDA1D(WignerMoyalCUDA1D):
             saved = self.rho.copy()
             saved_scale = self._scale_pending
             saved_ds = self._dscale.copy()
-            stats = empty((n, self.batch, 8), np.float64)
-            ring = empty((n, self.batch, len(record)), np.float64) if record else None
-            for s in range(n):
+            stats = fb["stats"] if fast else empty((n, self.batch, nc), np.float64)
+            ring = empty((n, self.batch, m), np.float64) if (record and not fast) else None
+
+            def step(s):
+                if fast:
+                    # [Re sum W, recorded sums, sum |rho|^2] and the scale, one launch
+                    self._normalized_step_sums(dbeta, g, m + 1, stats.ptr + s * self.batch * nc * 8)
+                    return
                 self._normalized_step(dbeta, stats[s])
                 if record:
-                    self._obs_launch(record, self.wignerfunction, float(self.t), ring.ptr + s * self.batch * len(record) * 8)
-            pur = self._purities(stats.get(), n)
-            bad = np.nonzero(~(pur < 1.))[0]
+                    self._obs_launch(record, self.wignerfunction, float(self.t), ring.ptr + s * self.batch * m * 8)
+
+            if fast:
+                graphs.run_chunk(fb["graphs"], gkey, lambda n=n: [step(s) for s in range(n)], n == CHECK_CHUNK)
+            else:
+                for s in range(n):
+                    step(s)
+            st = stats.get()[:n]
+            pur = (self._purities(st, n, nc, m + 1, float(self.P_gridDIM)) if fast
+                   else self._purities(st, n))
+            bad = np.nonzero(~(pur < self.purity_limit))[0]
             if len(bad):
                 stop = int(bad[0])
                 # replay up to and including the violating step (deterministic)
@@ -181,21 +238,35 @@ class WignerBlochCUDA1D(WignerMoyalCUDA1D):
                 self._scale_pending = saved_scale
                 self._dscale[:] = saved_ds
                 for s in range(stop + 1):
-                    self._normalized_step(dbeta, stats[s])
-                self._finish_normalization()
+                    step(s)
+                self._finish_gibbs(fast)
                 self.gibbs_stopped_at = k + stop
                 print("Warning: Gibbs state calculations interupted because purity = %.10f > 1."
                       "Current kT = %.6f" % (float(np.max(pur[stop])), 1 / (max(k + stop, 1) * dbeta)))
                 if record:
-                    rec.append(self._record_values(ring.get()[:stop + 1], stats.get()[:stop + 1]))
+                    rec.append(self._gibbs_record_values(fast, st, ring, m, stop + 1))
                 break
             if record:
-                rec.append(self._record_values(ring.get(), stats.get()))
+                rec.append(self._gibbs_record_values(fast, st, ring, m, n))
             k += n
-        self._finish_normalization()
+        self._finish_gibbs(fast)
         if record:
-            self.gibbs_record = np.concatenate(rec, axis=0) if rec else np.zeros((0, len(record)))
+            self.gibbs_record = np.concatenate(rec, axis=0) if rec else np.zeros((0, m))
         return self.wignerfunction
+
+    def _finish_gibbs(self, fast):
+        """Normalise rho and W by the last step's scale (W is formed here when the steps
+        did not form it)."""
+        if fast:
+            self.rho2wigner_()           # applies the pending scale to rho first
+        else:
+            self._finish_normalization()
+
+    def _gibbs_record_values(self, fast, st, ring, m, n):
+        if fast:
+            vals = st[:n, :, 1:1 + m] / st[:n, :, 0:1]
+            return vals[:, 0, :] if self.batch == 1 else vals
+        return self._record_values(ring.get()[:n], st[:n])
 
     def _record_values(self, ring, stats):
         # average = sum Re W f / Re sum W (the dXdP of W's normalisation cancels)

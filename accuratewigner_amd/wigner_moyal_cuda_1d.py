@@ -26,7 +26,7 @@ import numpy as np
 from . import rtc
 from ._params import KeywordParams
 from .device import DeviceArray, current_stream, empty, require_gpu, to_device
-from .engine import FWD, INV, Plan, mem_info, rank1_split, sum_partials
+from .engine import FWD, INV, Plan, finish_sums, mem_info, rank1_split, sum_partials
 from . import graphs, wm_ops
 from .wm_ops import WMGrid
 
@@ -256,20 +256,16 @@ __device__ __forceinline__ void aw_obs_body(const %(st)s *W, int n_item, double 
         const double P = dP * (i - 0.5 * P_gridDIM);
         const double w = (double)Wb[idx].x;
 %(body)s    }
-""" % dict(st=self._state_t(), nobs=nobs, body=body) + rtc.REDUCE_TAIL % dict(nobs=nobs) + "}\n"
-                   + rtc.REDUCE_FINISH % dict(nobs=nobs) + r"""
+""" % dict(st=self._state_t(), nobs=nobs, body=body) + rtc.REDUCE_TAIL % dict(nobs=nobs) + "}\n" + r"""
 extern "C" __global__ void aw_obs(const %(st)s *W, int n_item, double t, double *partials)
 {
     aw_obs_body(W, n_item, t, partials);
 }
 // the same with the time read from device memory (a captured step sequence replays with
-// new times written into the buffer), summed into out[item][:] by the last block, which
-// also writes the normalisation scale 1 / (out[item][0] * factor)
-extern "C" __global__ void aw_obs_fin(const %(st)s *W, int n_item, const double *tp, double *partials,
-                                      unsigned *counters, double *out, double factor, double *dscale)
+// new times written into the buffer)
+extern "C" __global__ void aw_obs_tp(const %(st)s *W, int n_item, const double *tp, double *partials)
 {
     aw_obs_body(W, n_item, *tp, partials);
-    aw_finish(partials, counters, out, factor, dscale);
 }
 // the expressions themselves on the grid at time t: out[k][i][j] = (f_k(X_j, P_i, t), 0)
 extern "C" __global__ void aw_obs_grid(double2 *out, int n_item, double t)
@@ -305,30 +301,34 @@ extern "C" __global__ void aw_obs_grid(double2 *out, int n_item, double t)
                                                           partials.ptr], current_stream())
         sum_partials(partials, nblk, len(exprs), self.batch, out)
 
-    def _obs_fin_launch(self, exprs, W, t_ptr, out_ptr, counters, factor, dscale):
-        """One launch: the sums of Re W * f_k(X, P, t) (t read from device memory at t_ptr)
-        into out[item][k] and the normalisation scale 1 / (out[item][0] factor) into dscale."""
+    def _obs_fin_launch(self, exprs, W, t_ptr, out_ptr, factor, dscale):
+        """The sums of Re W * f_k(X, P, t) (t read from device memory at t_ptr) into
+        out[item][k] and the normalisation scale 1 / (out[item][0] factor) into dscale: the
+        block partials, then one launch that adds them and derives the scale."""
         mod = self._observable_kernel(exprs)
         n = self.X_gridDIM * self.P_gridDIM
         nblk, partials = self._obs_partials_for(len(exprs))
-        mod.launch("aw_obs_fin", (nblk, self.batch), (256,),
-                   [W.ptr, ctypes.c_int(n), t_ptr, partials.ptr, counters.ptr, out_ptr,
-                    ctypes.c_double(factor), dscale.ptr], current_stream())
+        mod.launch("aw_obs_tp", (nblk, self.batch), (256,), [W.ptr, ctypes.c_int(n), t_ptr, partials.ptr],
+                   current_stream())
+        finish_sums(partials, nblk, len(exprs), self.batch, out_ptr, factor, dscale)
 
-    def _obs_weights(self, exprs):
+    def _obs_weights(self, exprs, t=None):
         """g_k = L^H f_k for observables f_k(X, P) that do not depend on t, with L = rho2wigner_
         (the unnormalised transform of wigner_moyal_cuda_1d.py:471-501): then
         sum Re(W) f_k = Re <f_k, L rho> = Re <g_k, rho>, one weighted sum over rho per step
         instead of the Wigner transform and the observable pass.  complex128 [nobs, Np, Nx],
-        or None when an expression may depend on t (or there are more than 8)."""
-        if len(exprs) > 8 or not all(time_independent(e, getattr(self, "functions", "")) for e in exprs):
+        or None when an expression may depend on t (or there are more than 8).  With t given,
+        the expressions are taken at that fixed time and may depend on it."""
+        if len(exprs) > 8:
+            return None
+        if t is None and not all(time_independent(e, getattr(self, "functions", "")) for e in exprs):
             return None
         Nx, Np = self.X_gridDIM, self.P_gridDIM
         mod = self._observable_kernel(exprs)
         g = empty((len(exprs), Np, Nx), np.complex128)
         blocks, threads = rtc.launch_cfg(Nx * Np)
-        mod.launch("aw_obs_grid", (blocks,), (threads,), [g.ptr, ctypes.c_int(Nx * Np), ctypes.c_double(0.0)],
-                   current_stream())
+        mod.launch("aw_obs_grid", (blocks,), (threads,),
+                   [g.ptr, ctypes.c_int(Nx * Np), ctypes.c_double(0.0 if t is None else float(t))], current_stream())
         self._obs_plan = Plan(Np, Nx, len(exprs), "fp64")   # lives with the system (its launches are queued)
         wm_ops.rho2wigner_adjoint(self._obs_plan, self._grid, g)
         return g
@@ -635,7 +635,7 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         CH = CHUNK_STEPS
         if self._ehr is None:
             self._ehr = dict(self._chunk_bufs(CH), stage=empty((CH, self.batch, nobs), np.float64),
-                             counters=to_device(np.zeros(self.batch, np.int32)), graphs={})
+                             graphs={})
             self._observable_kernel(exprs)
             self._obs_partials_for(nobs)             # allocated here, never inside a capture
             g = self._obs_weights(exprs) if self.obs_by_adjoint else None
@@ -706,7 +706,7 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             self.rho2wigner_()
             # observables into the stage row and the next scale 1 / (norm dXdP), one launch
             self._obs_fin_launch(exprs, self.wignerfunction, ts.ptr + 8 * k, stage.ptr + k * row_bytes,
-                                 bufs["counters"], self.dXdP, self._dscale)
+                                 self.dXdP, self._dscale)
             self._scale_pending = True
 
     def _ehrenfest_merged_chunk(self, exprs, n):

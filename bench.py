@@ -206,6 +206,31 @@ def main():
         lat_nr.append((time.perf_counter() - t0) * 1e3)
     lat_nr = np.array(lat_nr[2:])
 
+    # -- config 2: Wigner-Bloch Gibbs state, 512^2, x, p in [-8, 8), V = x^2/2 + 0.1 x^4,
+    #    beta 0 -> 2 in d beta = 0.005 (400 steps), renormalised and <H>(beta) recorded every
+    #    step; FP64 and mixed, 5 timed solves each --
+    from accuratewigner_amd import WignerBlochCUDA1D
+    gibbs, h_rec = {}, {}
+    for prec in ("fp64", "mixed"):
+        wb = WignerBlochCUDA1D(X_gridDIM=512, X_amplitude=8., P_gridDIM=512, P_amplitude=8., kT=0.5, dbeta=0.005,
+                               t_initial=0., t=0., V="0.5 * X * X + 0.1 * X * X * X * X", K="0.5 * P * P",
+                               precision=prec)
+        times = []
+        for r in range(6):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            wb.get_gibbs_state(record=[wb.hamiltonian])
+            torch.cuda.synchronize()
+            times.append((time.perf_counter() - t0) * 1e3)
+        h_rec[prec] = wb.gibbs_record[:, 0].copy()
+        p50 = float(np.percentile(times[1:], 50))
+        gibbs[prec] = {"p50_ms": p50, "p99_ms": float(np.percentile(times[1:], 99)),
+                       "gp_steps_per_s": 512 * 512 * len(h_rec[prec]) / (p50 * 1e-3)}
+        del wb
+    gibbs["mixed"]["H_rel_err_vs_fp64"] = float(np.max(np.abs(h_rec["mixed"] - h_rec["fp64"]) / np.abs(h_rec["fp64"])))
+    gibbs["workload"] = ("config 2: Wigner-Bloch Gibbs state 512x512, beta 0 -> 2 in 400 steps of 0.005, "
+                         "<H>(beta) and purity every step (5 timed solves)")
+
     serving = serving_run(args.serve_jobs) if args.serve_jobs > 0 else None
 
     traffic = None
@@ -248,6 +273,7 @@ def main():
                     "no_observables_mixed": {"workload": "WM mixed 256x256, 1000 steps, observables off (10 solves)",
                                              "p50_ms": float(np.percentile(lat_nr, 50)),
                                              "p99_ms": float(np.percentile(lat_nr, 99))}},
+        "gibbs": gibbs,
         "serving": serving,
         "cpu_baseline": cpu,
     }

@@ -180,17 +180,21 @@ int aw_pair_dot(aw_plan *plan, const void *a, const void *b, const double *u, co
 
 /*
  * Real parts of weighted sums of each batch item (FP64 accumulation, fixed order):
- *   out[b][q] = Re sum_ij conj(w[q][i][j]) data[b][i][j],   q < nw (1..8),
- * weights complex128 [nw][rows][cols], shared by every item; out [batch][nw] device FP64.
+ *   out[b][q] = Re sum_ij conj(w[q][i][j]) data[b][i][j],   q < nw,
+ * and with norm2 != 0 also out[b][nw] = sum_ij |data[b][i][j]|^2 (nw + norm2 <= 8);
+ * weights complex128 [nw][rows][cols], shared by every item; out [batch][nw + norm2] FP64.
  * With dscale != NULL also dscale[b] = 1 / (out[b][0] * factor) (complex, imaginary part 0),
- * the normalisation of the next step.  One launch (the last block of each item adds the
- * block partials).  The propagators record the reference's per-step Ehrenfest averages
+ * the normalisation of the next step.  Two launches (block partials, then one block per
+ * item adds them).  The propagators record the reference's per-step Ehrenfest averages
  * sum Re W f_q dXdP (wigner_moyal_cuda_1d.py:584-606, W from rho2wigner_ :471-501) as
- * Re <g_q, rho> with g_q = rho2wigner^H f_q when f_q does not depend on t.  The first call
+ * Re <g_q, rho> with g_q = rho2wigner^H f_q when f_q does not depend on t; the Gibbs loop's
+ * purity check 2 pi Re sum W^2 dXdP < 1 (wigner_bloch_cuda_1d.py:139-147 with get_purity,
+ * wigner_moyal_cuda_1d.py:349-354) from
+ * sum W^2 = Np sum |rho|^2 (the transform is Np times an isometry).  The first call
  * on a plan allocates its scratch and must not be made under stream capture; calls on one
  * plan must not overlap.
  */
-int aw_weighted_sums(aw_plan *plan, const void *data, const double *weights, int nw, double *out,
+int aw_weighted_sums(aw_plan *plan, const void *data, const double *weights, int nw, int norm2, double *out,
                      double factor, double *dscale, aw_stream stream);
 
 /*
@@ -226,6 +230,15 @@ int aw_cyclic_diag_sums(aw_plan *plan, const void *data, double *out, aw_stream 
  */
 int aw_sum_partials(const double *partials, int nblk, int ncomp, int batch, double *out,
                     aw_stream stream);
+
+/*
+ * The same sums (ncomp <= 8) added by a whole block per item in a fixed order, and with
+ * dscale != NULL the normalisation scale dscale[b] = 1 / (out[b][0] * factor) in the same
+ * launch: sum_partials followed by scale_from_stats (mode 0) of the reference's per-step
+ * normalisation, e.g. wigner_moyal_cuda_1d.py:608-622 after the Ehrenfest sums (:584-606).
+ */
+int aw_finish_sums(const double *partials, int nblk, int ncomp, int batch, double *out, double factor,
+                   double *dscale, aw_stream stream);
 
 /* Same statistics for an arbitrary complex vector of length n per item (for 1D states). */
 int aw_reduce_vec(int prec, const void *data, long long n, int batch, double *out,

@@ -67,3 +67,39 @@ def test_time_dependent_observables_keep_the_transform():
     s = _run(True, V="0.5 * X * X + X * 0.1 * sin(0.5 * t)", diff_V="X + 0.1 * sin(0.5 * t)", steps=20)
     assert s._ehr["weights"] is None
     assert len(s.X_average) == 20
+
+
+def _gibbs(by_adjoint, kT, V="0.5 * X * X + 0.1 * X * X * X * X", record=("X * X",), dbeta=0.005, N=128,
+           purity_limit=1.):
+    from accuratewigner_amd import WignerBlochCUDA1D
+    g = WignerBlochCUDA1D(X_gridDIM=N, X_amplitude=8., P_gridDIM=N, P_amplitude=8., kT=kT, dbeta=dbeta,
+                          t_initial=0., t=0., V=V, K="0.5 * P * P")
+    g.obs_by_adjoint = by_adjoint
+    g.purity_limit = purity_limit
+    W = g.get_gibbs_state(record=[g.hamiltonian] + list(record)).get()
+    return g, W
+
+
+def test_gibbs_loop_sums_without_the_transform_match_fp64():
+    """Config-2 shape at 128^2: the Gibbs loop's norm, recorded averages and purity taken
+    from rho (adjoint weights, Parseval) equal the loop that forms W every step."""
+    a, Wa = _gibbs(True, 0.5)
+    b, Wb = _gibbs(False, 0.5)
+    assert a.gibbs_record.shape == (400, 2) and a.gibbs_stopped_at is None and b.gibbs_stopped_at is None
+    assert np.max(np.abs(a.gibbs_record - b.gibbs_record) / np.abs(b.gibbs_record)) < 1e-12
+    assert np.linalg.norm(Wa - Wb) < 1e-12 * np.linalg.norm(Wb)
+    assert abs(np.sum(Wa.real) * a.dXdP - 1.) < 1e-12
+
+
+def test_gibbs_purity_guard_with_sums_from_rho():
+    """The purity guard (stop when the purity is not below the limit, replay up to that
+    step, normalise) on both paths: a harmonic Gibbs run whose purity passes 0.5 on the way
+    to tanh(1) = 0.76 stops at the same step, with the same state."""
+    a, Wa = _gibbs(True, 0.5, V="0.5 * X * X", record=(), purity_limit=0.5)
+    b, Wb = _gibbs(False, 0.5, V="0.5 * X * X", record=(), purity_limit=0.5)
+    assert a.gibbs_stopped_at is not None and a.gibbs_stopped_at == b.gibbs_stopped_at
+    assert 0 < a.gibbs_stopped_at < 399
+    assert len(a.gibbs_record) == a.gibbs_stopped_at + 1
+    assert abs(np.sum(Wa.real) * a.dXdP - 1.) < 1e-12
+    assert np.linalg.norm(Wa - Wb) < 1e-12 * np.linalg.norm(Wb)
+    assert 0.5 <= a.get_purity() < 0.51

@@ -87,13 +87,14 @@ def test_entry_points_reject_null_and_bad_arguments_before_touching_the_gpu():
         (_lib.awigInvalidPlan, lambda: lib.aw_step_group(None, 4, 0, ctypes.byref(ctypes.c_int()))),
         (_lib.awigInvalidPlan, lambda: lib.aw_pair_dot(None, vp(1), vp(1), vp(1), vp(1), vp(1), None)),
         (_lib.awigInvalidPlan, lambda: lib.aw_cyclic_diag_sums(None, vp(1), vp(1), None)),
-        (_lib.awigInvalidPlan, lambda: lib.aw_weighted_sums(None, vp(1), vp(1), 1, vp(1), 1., None, None)),
+        (_lib.awigInvalidPlan, lambda: lib.aw_weighted_sums(None, vp(1), vp(1), 1, 0, vp(1), 1., None, None)),
         (_lib.awigInvalidPlan, lambda: lib.aw_cyclic_diag_obs(None, vp(1), vp(1), 3, vp(1), 3, vp(1), 8, 1.,
                                                                None, None)),
         (_lib.awigInvalidPlan, lambda: lib.aw_reduce_stats(None, vp(1), vp(1), None)),
         (_lib.awigInvalidPlan, lambda: lib.aw_make_factors(None, vp(1), 0, 8, 1, 0, None, 8, 0., None, 8, 0.,
                                                             0., None, 0, 1, 1., 0, None)),
         (_lib.awigInvalidValue, lambda: lib.aw_sum_partials(None, 1, 1, 1, vp(1), None)),
+        (_lib.awigInvalidValue, lambda: lib.aw_finish_sums(vp(1), 1, 9, 1, vp(1), 1., None, None)),
         (_lib.awigInvalidValue, lambda: lib.aw_scale_from_stats(None, 8, 0, 0, 1., vp(1), 1, None)),
         (_lib.awigInvalidValue, lambda: lib.aw_project_out(0, vp(1), None, 1, 128, None)),
         (_lib.awigInvalidValue, lambda: lib.aw_project_out(0, vp(1), vp(1), 5000, 128, None)),
