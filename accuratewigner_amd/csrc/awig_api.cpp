@@ -353,6 +353,26 @@ static int ensure_finish_scratch(aw_plan *p, aw_stream stream, const char *who) 
     return AW_OK;
 }
 
+// blocks per item of the weighted sums: 8 elements per thread (4 independent loads in
+// flight), at most kWsumMaxBlocks
+static int wsum_blocks(const aw_plan *p) {
+    const long long nb = (long long)p->rows * p->cols / 2048;
+    return int(nb < 1 ? 1 : (nb > kWsumMaxBlocks ? kWsumMaxBlocks : nb));
+}
+
+int aw_weighted_partials(aw_plan *p, const void *data, const double *weights, int nw, int norm2,
+                         double *partials, int *nblk, aw_stream stream) {
+    if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
+    if (nw < 1 || nw + (norm2 ? 1 : 0) > 8)
+        return set_error(AW_ERR_INVALID_VALUE, "nw must be 1..%d (got %d)", norm2 ? 7 : 8, nw);
+    const int nb = wsum_blocks(p);
+    if (nblk) *nblk = nb;
+    if (!partials) return AW_OK;   // a query of the block count
+    if (!data || !weights) return set_error(AW_ERR_INVALID_VALUE, "NULL data or weights");
+    return launch_weighted_partials(p->prec, data, (long long)p->rows * p->cols, weights, nw, norm2 ? 1 : 0,
+                                    p->batch, partials, nb, (hipStream_t)stream);
+}
+
 int aw_weighted_sums(aw_plan *p, const void *data, const double *weights, int nw, int norm2, double *out,
                      double factor, double *dscale, aw_stream stream) {
     if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
@@ -362,9 +382,7 @@ int aw_weighted_sums(aw_plan *p, const void *data, const double *weights, int nw
     const int rc = ensure_finish_scratch(p, stream, "aw_weighted_sums");
     if (rc != AW_OK) return rc;
     const long long n_item = (long long)p->rows * p->cols;
-    // 8 elements per thread (4 independent loads in flight), at most 512 blocks
-    const long long nb = n_item / 2048;
-    const int nblk = int(nb < 1 ? 1 : (nb > kWsumMaxBlocks ? kWsumMaxBlocks : nb));
+    const int nblk = wsum_blocks(p);
     return launch_weighted_sums(p->prec, data, n_item, weights, nw, norm2 ? 1 : 0, p->batch, p->wsum_partials,
                                 nblk, out, factor, dscale, (hipStream_t)stream);
 }

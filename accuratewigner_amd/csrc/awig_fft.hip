@@ -315,15 +315,22 @@ static int launch_wsums_r(const void *data, long long n_item, const double *w, i
     return AW_OK;
 }
 
-int launch_weighted_sums(int prec, const void *data, long long n_item, const double *w, int nw, int norm2,
-                         int batch, double *partials, int nblk, double *out, double factor, double *dscale,
-                         hipStream_t st) {
+int launch_weighted_partials(int prec, const void *data, long long n_item, const double *w, int nw, int norm2,
+                             int batch, double *partials, int nblk, hipStream_t st) {
     const int rc = prec == AW_PREC_F64
                        ? launch_wsums_r<double>(data, n_item, w, nw, norm2, batch, partials, nblk, st)
                        : launch_wsums_r<float>(data, n_item, w, nw, norm2, batch, partials, nblk, st);
     if (rc != AW_OK) return rc;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_weighted_sums: %s", hipGetErrorString(e));
+    return AW_OK;
+}
+
+int launch_weighted_sums(int prec, const void *data, long long n_item, const double *w, int nw, int norm2,
+                         int batch, double *partials, int nblk, double *out, double factor, double *dscale,
+                         hipStream_t st) {
+    const int rc = launch_weighted_partials(prec, data, n_item, w, nw, norm2, batch, partials, nblk, st);
+    if (rc != AW_OK) return rc;
     return launch_finish_sums(partials, nblk, nw + norm2, batch, out, factor, dscale, st);
 }
 

@@ -60,6 +60,8 @@ int launch_weighted_sums(int prec, const void *data, long long n_item, const dou
                          hipStream_t st);
 int launch_finish_sums(const double *partials, int nblk, int ncomp, int batch, double *out, double factor,
                        double *dscale, hipStream_t st);
+int launch_weighted_partials(int prec, const void *data, long long n_item, const double *w, int nw, int norm2,
+                             int batch, double *partials, int nblk, hipStream_t st);
 int launch_cyclic_obs(int prec, const void *data, int n, int batch, double *work, const double *xw, int nx,
                       const double *pw_hat, int np, double *partials, unsigned *counters, double *out,
                       long long out_bstride, double factor, double *dscale, hipStream_t st);

@@ -164,6 +164,15 @@ class Plan:
                   ctypes.c_void_p(current_stream()))
         return out
 
+    def weighted_partials(self, data, weights, nw, partials=None, norm2=False):
+        """Block partials [batch, nblk, nw (+1)] of weighted_sums into ``partials`` (a device
+        pointer or array); returns nblk (partials None: only the block count)."""
+        nb = ctypes.c_int()
+        _lib.call("aw_weighted_partials", self.handle, ctypes.c_void_p(_p(data)), ctypes.c_void_p(_p(weights)),
+                  int(nw), int(bool(norm2)), ctypes.c_void_p(_p(partials)), ctypes.byref(nb),
+                  ctypes.c_void_p(current_stream()))
+        return nb.value
+
     def cyclic_diag_obs(self, data, xw, nx, pw_hat, np_, out, out_bstride, factor=0.0, dscale=None):
         """out[b, :1 + nx + np_] = trace, coordinate sums over diag rho (FP64 weights xw
         [nx, n]) and momentum sums Re sum_d c_d pw_hat[q, d] / n from the signed cyclic

@@ -643,15 +643,21 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             # them): the steps are merged and the observables read from each row-pass output
             self._ehr["weights"] = self._closing_adjoint(g) if g is not None else None
             if g is not None:
-                # the first call allocates the plan's scratch (never inside a capture)
-                self.plan.weighted_sums(self.rho, self._ehr["weights"], nobs, self._ehr["stage"])
+                # block partials of every step of a chunk, added by one launch per chunk
+                nblk = self.plan.weighted_partials(None, None, nobs)
+                self._ehr["wring"] = empty((CH, self.batch, nblk, nobs), np.float64)
+                self._ehr["wnblk"] = nblk
         bufs = self._ehr
-        if not self._scale_pending:
+        merged = bufs["weights"] is not None
+        if merged:
+            # the steps are linear, so the reference's per-step normalisation (:584-622) is
+            # applied once at the end; the recorded averages are ratios to the recorded norm
+            self._flush_scale()
+        elif not self._scale_pending:
             # every step applies the pending scale: start from 1 so that all chunks are alike
             self._dscale.tensor[:, 0] = 1.0
             self._dscale.tensor[:, 1] = 0.0
             self._scale_pending = True
-        merged = bufs["weights"] is not None
         chunk = self._ehrenfest_merged_chunk if merged else self._ehrenfest_chunk
         for c0 in range(0, steps, CH):
             tsc = ts[c0:c0 + CH]
@@ -661,8 +667,8 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             ring.tensor[c0:c0 + n].copy_(bufs["stage"].tensor[:n])
         self.t = ts[-1]
         # final normalisation of rho and W by the last step's norm
-        if bufs["weights"] is not None:
-            self.rho2wigner_()           # W of the normalised rho (not formed during the steps)
+        if merged:
+            self.normalize_rho_wigner()  # W (not formed during the steps), then W and rho / norm
         else:
             self.plan.scale(self.wignerfunction, self._dscale)
             self._flush_scale()
@@ -714,9 +720,10 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         like the plain loop: [EV_0 -> I0] ([F1 -> EK_s -> I1] [sums] [F0 -> EV EV -> I0])*
         ending [F0 -> EV].  The step-complete rho_s = C1 F0(Y_s), with Y_s the row-pass output
         and C1 the closing half-kick, so its observables Re <g, rho_s> = Re <g', Y_s> with
-        g' = I0(conj(C1) g) (``_closing_adjoint``); the sums also give the normalisation,
-        which the next merged column pass applies.  Two passes and one reduction per step.
-        Only launches, from fixed addresses."""
+        g' = I0(conj(C1) g) (``_closing_adjoint``).  Each step leaves block partials in a
+        ring; one launch per chunk adds them into the stage rows.  No per-step
+        normalisation: the steps are linear and the averages are ratios to the recorded norm.
+        Two passes and one reduction launch per step.  Only launches, from fixed addresses."""
         bufs, p, rho = self._ehr, self.plan, self.rho
         colL, colE, rowL, rowE = bufs["fac"]
         _, _, tab1 = self._mask_args(0, 1)
@@ -724,19 +731,18 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         _, _, tabp = self._mask_args(1, 1)
         csz = np.dtype(self.state_dtype).itemsize
         Nx, Np = self.X_gridDIM, self.P_gridDIM
-        nobs = len(exprs)
-        row_bytes = self.batch * nobs * 8
-        stage, gy = bufs["stage"], bufs["weights"]
-        p.run(rho, 0, 0, INV, line_fac=colL, elem_fac=colE, mask=tab1, dscale=self._dscale)
+        nobs, nblk = len(exprs), bufs["wnblk"]
+        gy, ring = bufs["weights"], bufs["wring"]
+        ring_row = self.batch * nblk * nobs * 8
+        p.run(rho, 0, 0, INV, line_fac=colL, elem_fac=colE, mask=tab1)
         for k in range(n):
             p.run(rho, 1, FWD, INV, line_fac=rowL.ptr + k * Np * csz, elem_fac=rowE.ptr + k * Nx * csz,
                   mask=tabp)
-            p.weighted_sums(rho, gy, nobs, stage.ptr + k * row_bytes, self.dXdP, self._dscale)
+            p.weighted_partials(rho, gy, nobs, ring.ptr + k * ring_row)
             last = k == n - 1
             p.run(rho, 0, FWD, 0 if last else INV, line_fac=colL.ptr + (k + 1) * Nx * csz,
-                  elem_fac=colE.ptr + (k + 1) * Np * csz, mask=tab1 if last else tab2,
-                  dscale=None if last else self._dscale)
-        self._scale_pending = True
+                  elem_fac=colE.ptr + (k + 1) * Np * csz, mask=tab1 if last else tab2)
+        finish_sums(ring, nblk, nobs, n * self.batch, bufs["stage"])
 
     def _closing_adjoint(self, g):
         """g' = I0(conj(C1) g) for each weight array g_q (FP64): C1 is the closing half-kick of

@@ -198,6 +198,18 @@ int aw_weighted_sums(aw_plan *plan, const void *data, const double *weights, int
                      double factor, double *dscale, aw_stream stream);
 
 /*
+ * The first stage of aw_weighted_sums alone: block partials
+ * partials[b][k][c] (c < nw + norm2, k < *nblk) into a caller buffer, one launch, no scratch
+ * (so any call may be captured).  *nblk receives the block count (partials = NULL: only that).
+ * aw_finish_sums over several such buffers laid end to end adds many steps' sums in one launch:
+ * the propagators record a chunk of steps this way (the sums are ratios, so the per-step
+ * normalisation of the reference, wigner_moyal_cuda_1d.py:608-622, commutes with the linear
+ * steps and is applied once at the end).
+ */
+int aw_weighted_partials(aw_plan *plan, const void *data, const double *weights, int nw, int norm2,
+                         double *partials, int *nblk, aw_stream stream);
+
+/*
  * Per-step Ehrenfest sums of square density matrices (n >= 64), FP64, two launches:
  *   out[b][0]          = Re tr rho[b],
  *   out[b][1 + q]      = sum_i Re rho[b][i][i] xw[q][i]                  (q < nx),

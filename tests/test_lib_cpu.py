@@ -88,6 +88,7 @@ def test_entry_points_reject_null_and_bad_arguments_before_touching_the_gpu():
         (_lib.awigInvalidPlan, lambda: lib.aw_pair_dot(None, vp(1), vp(1), vp(1), vp(1), vp(1), None)),
         (_lib.awigInvalidPlan, lambda: lib.aw_cyclic_diag_sums(None, vp(1), vp(1), None)),
         (_lib.awigInvalidPlan, lambda: lib.aw_weighted_sums(None, vp(1), vp(1), 1, 0, vp(1), 1., None, None)),
+        (_lib.awigInvalidPlan, lambda: lib.aw_weighted_partials(None, vp(1), vp(1), 1, 0, None, None, None)),
         (_lib.awigInvalidPlan, lambda: lib.aw_cyclic_diag_obs(None, vp(1), vp(1), 3, vp(1), 3, vp(1), 8, 1.,
                                                                None, None)),
         (_lib.awigInvalidPlan, lambda: lib.aw_reduce_stats(None, vp(1), vp(1), None)),
