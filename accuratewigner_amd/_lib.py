@@ -82,6 +82,7 @@ class aw_pass(ctypes.Structure):
         ("lfftorder", ctypes.c_int), ("efftorder", ctypes.c_int),
         ("scale", ctypes.c_double), ("dscale", ctypes.c_void_p), ("out", ctypes.c_void_p),
         ("ctab", ctypes.c_void_p), ("ctab_bstride", ctypes.c_longlong), ("chirp_out", ctypes.c_double),
+        ("obs_weights", ctypes.c_void_p), ("obs_nw", ctypes.c_int), ("obs_partials", ctypes.c_void_p),
     ]
 
 
@@ -108,6 +109,7 @@ _SIGNATURES = {
     "aw_plan_destroy": [_vp],
     "aw_fft_axis": [_vp, _vp, _vp, _i, _i, _vp],
     "aw_pass_run": [_vp, _vp, ctypes.POINTER(aw_pass), _vp],
+    "aw_row_obs_blocks": [_vp, ctypes.POINTER(_i)],
     "aw_step_sequence": [_vp, _vp, ctypes.POINTER(aw_steps), _vp],
     "aw_step_group": [_vp, _i, _i, ctypes.POINTER(_i)],
     "aw_make_factors": [_vp, _vp, _ll, _i, _i, _i, _vp, _ll, _d, _vp, _ll, _d, _d, _vp, _ll, _i,

@@ -171,10 +171,26 @@ static int pass_run_n(aw_plan *p, void *data, const aw_pass *ps, int nbatch, aw_
     a.ctab = ps->ctab;
     a.c_bstride = ps->ctab_bstride;
     a.chirp_out = ps->chirp_out;
+    if (ps->obs_weights) {
+        if (ps->obs_nw < 1 || ps->obs_nw > 8 || !ps->obs_partials)
+            return set_error(AW_ERR_INVALID_VALUE, "obs_nw must be 1..8 with an obs_partials buffer");
+        a.obs_w = static_cast<const double2 *>(ps->obs_weights);
+        a.obs_nw = ps->obs_nw;
+        a.obs_partials = ps->obs_partials;
+    }
     a.has_factor = (ps->line_fac || ps->elem_fac || ps->mask || ps->ctab || ps->chirp != 0.0 ||
                     ps->scale != 1.0 || ps->dscale) ? 1 : 0;
     a.tw = strided ? p->tw_rows : p->tw_cols;
     return launch_pass(p->prec, n, strided, a, (hipStream_t)stream);
+}
+
+int aw_row_obs_blocks(const aw_plan *p, int *nblk) {
+    if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
+    if (!nblk) return set_error(AW_ERR_INVALID_VALUE, "NULL nblk");
+    const int lpb = row_obs_lpb(p->prec, p->cols, (long long)p->rows * p->batch);
+    if (lpb < 1) return set_error(AW_ERR_INVALID_SIZE, "unsupported row length %d", p->cols);
+    *nblk = p->rows / lpb;
+    return AW_OK;
 }
 
 int aw_pass_run(aw_plan *p, void *data, const aw_pass *ps, aw_stream stream) {

@@ -44,6 +44,12 @@ int set_error(int code, const char *fmt, ...) {
 // ---------------------------------------------------------------------------------------
 int launch_pass_f32(int n, bool strided, int kind, const PassArgs &a, hipStream_t st);
 int launch_pass_f64(int n, bool strided, int kind, const PassArgs &a, hipStream_t st);
+int row_obs_lpb_f32(int n, long long total_lines);
+int row_obs_lpb_f64(int n, long long total_lines);
+
+int row_obs_lpb(int prec, int n, long long total_lines) {
+    return prec == AW_PREC_F64 ? row_obs_lpb_f64(n, total_lines) : row_obs_lpb_f32(n, total_lines);
+}
 
 int launch_pass(int prec, int n, bool strided, const PassArgs &a, hipStream_t st) {
     int kind = 0;
@@ -53,6 +59,11 @@ int launch_pass(int prec, int n, bool strided, const PassArgs &a, hipStream_t st
         if (a.dir_a == 0 && a.dir_b == AW_INVERSE) kind = 1;
         else if (a.dir_a == AW_FORWARD && a.dir_b == AW_INVERSE) kind = 2;
         else if (a.dir_a == AW_FORWARD && a.dir_b == 0) kind = 3;
+    }
+    if (a.obs_w) {
+        if (kind != 2 || strided)
+            return set_error(AW_ERR_INVALID_VALUE, "weighted-sum partials need a plain [forward, factor, inverse] row pass");
+        kind = 4;
     }
     return prec == AW_PREC_F64 ? launch_pass_f64(n, strided, kind, a, st)
                                : launch_pass_f32(n, strided, kind, a, st);

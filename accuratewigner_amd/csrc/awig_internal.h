@@ -33,6 +33,9 @@ struct PassArgs {
     const void *tw;         // N roots exp(-2 pi i t / N), plan precision
     const void *ctab; long long c_bstride;   // complex per-element factor table (general passes)
     double chirp_out;                        // exp(i chirp_out lcoord ecoord) after the second FFT
+    const double2 *obs_w;   // row passes: complex128 weights [obs_nw][rows][cols] shared by the items
+    int obs_nw;             // 1..8
+    double *obs_partials;   // [batch][nlines / LPB][obs_nw]: Re sum conj(w) out over each block
 };
 
 struct FactorArgs {
@@ -47,6 +50,8 @@ struct FactorArgs {
 };
 
 int launch_pass(int prec, int n, bool strided, const PassArgs &a, hipStream_t st);
+// lines per block of the row step pass with weighted-sum partials (axis 1, line length n)
+int row_obs_lpb(int prec, int n, long long total_lines);
 int launch_make_factors(int prec, const FactorArgs &f, hipStream_t st);
 int launch_stats(int prec, const void *data, long long n_item, int rows, int cols, int batch,
                  double *partials, int nblk, double *out, hipStream_t st);

@@ -60,13 +60,13 @@ template <typename R, int N, bool STRIDED> struct SmallTune {
     static constexpr int LPB = STRIDED ? 2 : (T >= 64 ? 1 : 64 / T);
 };
 
-template <typename R, int N, bool STRIDED, int DA, int DB, int GEN>
+template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0>
 using PassK = KCfg<R, N, PassTune<R, N, STRIDED>::E, PassTune<R, N, STRIDED>::LPB, STRIDED,
-                   PassTune<R, N, STRIDED>::TWM, DA, DB, GEN>;
+                   PassTune<R, N, STRIDED>::TWM, DA, DB, GEN, 1, 0, 4, 5, OBS>;
 
-template <typename R, int N, bool STRIDED, int DA, int DB, int GEN>
+template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0>
 using PassKS = KCfg<R, N, SmallTune<R, N, STRIDED>::E, SmallTune<R, N, STRIDED>::LPB, STRIDED, 1, DA,
-                    DB, GEN>;
+                    DB, GEN, 1, 0, 4, 5, OBS>;
 
 template <typename C>
 int launch_cfg(const PassArgs &a, hipStream_t st) {
@@ -81,12 +81,16 @@ int launch_cfg(const PassArgs &a, hipStream_t st) {
 }
 
 // kind: 0 general (run-time directions, signs, chirps, diagonal output);
-//       1 step [factor, inverse FFT]; 2 step [forward, factor, inverse]; 3 step [forward, factor]
+//       1 step [factor, inverse FFT]; 2 step [forward, factor, inverse]; 3 step [forward, factor];
+//       4 row step [forward, factor, inverse] that also stores weighted-sum block partials
 template <typename R, int N, bool STRIDED>
 int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
     if constexpr (N <= 2048) {
         if (a.total_lines <= small_lines_max<R, N>()) {
             switch (kind) {
+            case 4:
+                if constexpr (!STRIDED) return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0, 1>>(a, st);
+                return set_error(AW_ERR_INVALID_VALUE, "weighted-sum partials need a row pass");
             case 1: return launch_cfg<PassKS<R, N, STRIDED, 0, 1, 0>>(a, st);
             case 2: return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0>>(a, st);
             case 3: return launch_cfg<PassKS<R, N, STRIDED, -1, 0, 0>>(a, st);
@@ -95,10 +99,34 @@ int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
         }
     }
     switch (kind) {
+    case 4:
+        if constexpr (!STRIDED) return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0, 1>>(a, st);
+        return set_error(AW_ERR_INVALID_VALUE, "weighted-sum partials need a row pass");
     case 1: return launch_cfg<PassK<R, N, STRIDED, 0, 1, 0>>(a, st);
     case 2: return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0>>(a, st);
     case 3: return launch_cfg<PassK<R, N, STRIDED, -1, 0, 0>>(a, st);
     default: return launch_cfg<PassK<R, N, STRIDED, 2, 2, 1>>(a, st);
+    }
+}
+
+// lines per block of the row pass that launch_kind_n picks for total_lines lines of n points
+template <typename R, int N> long long row_lpb_n(long long total_lines) {
+    if constexpr (N <= 2048) {
+        if (total_lines <= small_lines_max<R, N>()) return SmallTune<R, N, false>::LPB;
+    }
+    return PassTune<R, N, false>::LPB;
+}
+
+template <typename R> int row_lpb(int n, long long total_lines) {
+    switch (n) {
+    case 64: return int(row_lpb_n<R, 64>(total_lines));
+    case 128: return int(row_lpb_n<R, 128>(total_lines));
+    case 256: return int(row_lpb_n<R, 256>(total_lines));
+    case 512: return int(row_lpb_n<R, 512>(total_lines));
+    case 1024: return int(row_lpb_n<R, 1024>(total_lines));
+    case 2048: return int(row_lpb_n<R, 2048>(total_lines));
+    case 4096: return int(row_lpb_n<R, 4096>(total_lines));
+    default: return 0;
     }
 }
 

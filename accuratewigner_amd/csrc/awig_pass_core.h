@@ -228,10 +228,11 @@ __device__ __forceinline__ void fft_line(int dir, T2 *v, T2 *L, int l, int u, co
 // passes (GEN = 0) only multiply by line x element factor vectors (and an optional mask),
 // which keeps their code to two FFT bodies.
 template <typename R_, int N_, int E_, int LPB_, bool STRIDED_, int TWM_, int DA_ = 2, int DB_ = 2,
-          int GEN_ = 1, int MINW_ = 1, int PF_ = 0, int PADC_ = 4, int PADR_ = 5>
+          int GEN_ = 1, int MINW_ = 1, int PF_ = 0, int PADC_ = 4, int PADR_ = 5, int OBS_ = 0>
 struct KCfg {
     using R = R_;
     static constexpr int N = N_, E = E_, LPB = LPB_, TWM = TWM_, DA = DA_, DB = DB_, GEN = GEN_;
+    static constexpr int OBS = OBS_;     // 1: row passes also store weighted-sum block partials
     static constexpr int MINW = MINW_;   // minimum waves per SIMD the register budget must allow
     static constexpr int PF = PF_;       // 1: load the factor vectors with the data, before the first FFT
     static constexpr bool STRIDED = STRIDED_;
@@ -400,6 +401,42 @@ void k_line_pass(PassArgs a) {
             const int row = STRIDED ? e : line;
             const int par = a.sign_out == 2 ? row : (line + e);
             if (par & 1) v[k] = cscale(v[k], R(-1));
+        }
+    }
+    if constexpr (C::OBS) {
+        // block partials of Re sum conj(w_q) out over this block's lines (all of one item):
+        // each thread its elements, then the waves, then the waves of the block
+        static_assert(!STRIDED && C::THREADS % 64 == 0, "row passes of whole waves");
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (active) {
+            const long long nit = (long long)N * nl;
+            const double2 *w = a.obs_w + off0;   // the weights are shared by the items
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const double x = v[k].x, y = v[k].y;
+                for (int q = 0; q < a.obs_nw; ++q) {
+                    const double2 c = w[q * nit + (long long)k * kstep];
+                    acc[q] += c.x * x + c.y * y;
+                }
+            }
+        }
+        __shared__ double ored[C::THREADS / 64][8];
+        const int lane = tid & 63, wid = tid >> 6;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            double s = acc[q];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+            if (lane == 0) ored[wid][q] = s;
+        }
+        __syncthreads();
+        if (tid < a.obs_nw) {
+            double s = 0.0;
+            for (int q = 0; q < C::THREADS / 64; ++q) s += ored[q][tid];
+            const long long g0 = blk * LPB;
+            const long long bb = g0 / nl;
+            const long long bi = (g0 - bb * nl) / LPB;
+            a.obs_partials[(bb * (nl / LPB) + bi) * a.obs_nw + tid] = s;
         }
     }
     if (!active) return;

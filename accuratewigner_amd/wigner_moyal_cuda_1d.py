@@ -644,7 +644,7 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             self._ehr["weights"] = self._closing_adjoint(g) if g is not None else None
             if g is not None:
                 # block partials of every step of a chunk, added by one launch per chunk
-                nblk = self.plan.weighted_partials(None, None, nobs)
+                nblk = self.plan.row_obs_blocks()
                 self._ehr["wring"] = empty((CH, self.batch, nblk, nobs), np.float64)
                 self._ehr["wnblk"] = nblk
         bufs = self._ehr
@@ -723,7 +723,8 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         g' = I0(conj(C1) g) (``_closing_adjoint``).  Each step leaves block partials in a
         ring; one launch per chunk adds them into the stage rows.  No per-step
         normalisation: the steps are linear and the averages are ratios to the recorded norm.
-        Two passes and one reduction launch per step.  Only launches, from fixed addresses."""
+        Two launches per step: the row pass stores the partials as it writes Y.  Only
+        launches, from fixed addresses."""
         bufs, p, rho = self._ehr, self.plan, self.rho
         colL, colE, rowL, rowE = bufs["fac"]
         _, _, tab1 = self._mask_args(0, 1)
@@ -736,9 +737,9 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         ring_row = self.batch * nblk * nobs * 8
         p.run(rho, 0, 0, INV, line_fac=colL, elem_fac=colE, mask=tab1)
         for k in range(n):
+            # the row pass also leaves the block partials of Re <g', Y_k> in the ring
             p.run(rho, 1, FWD, INV, line_fac=rowL.ptr + k * Np * csz, elem_fac=rowE.ptr + k * Nx * csz,
-                  mask=tabp)
-            p.weighted_partials(rho, gy, nobs, ring.ptr + k * ring_row)
+                  mask=tabp, obs_weights=gy, obs_nw=nobs, obs_partials=ring.ptr + k * ring_row)
             last = k == n - 1
             p.run(rho, 0, FWD, 0 if last else INV, line_fac=colL.ptr + (k + 1) * Nx * csz,
                   elem_fac=colE.ptr + (k + 1) * Np * csz, mask=tab1 if last else tab2)

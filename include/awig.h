@@ -97,9 +97,18 @@ typedef struct aw_pass {
     void *out;
     const void *ctab; long long ctab_bstride;         /* complex [rows][cols] factor table */
     double chirp_out;
+    /* plain [forward, factor, inverse] row passes only: block partials of the weighted sums
+     * Re sum conj(obs_weights[q]) out (q < obs_nw <= 8; complex128 [obs_nw][rows][cols],
+     * shared by the items) into obs_partials[b][k][q], k < aw_row_obs_blocks(), to be added
+     * by aw_finish_sums.  NULL weights: none. */
+    const void *obs_weights; int obs_nw;
+    double *obs_partials;
 } aw_pass;
 
 int aw_pass_run(aw_plan *plan, void *data, const aw_pass *pass, aw_stream stream);
+
+/* Blocks per item (k range) of the obs_partials of a row pass over the plan's batch. */
+int aw_row_obs_blocks(const aw_plan *plan, int *nblk);
 
 /*
  * n merged Strang steps of a 2D split-operator propagator in 2n + 1 fused passes:
