@@ -10,7 +10,9 @@ constexpr int kLdsBudget = 160 * 1024;
 // Per (precision, line length, line kind):
 //  * E = 16 elements per thread (8 below 256), so a 1024-point line is 64 lanes, radix 16-16-4;
 //    FP32 columns of 512 and 1024 points use E = 32 (radix 32-16 / 32-32: one LDS exchange,
-//    8 columns per 128 / 256-thread block; 14% faster at 1024^2 x 96 than E = 16);
+//    8 columns per 128 / 256-thread block; 14% faster at 1024^2 x 96 than E = 16), and
+//    exchange the real and the imaginary parts in turn (43 instead of 76 KB of LDS; 6% faster
+//    on a MALL-resident 24-state group of 1024^2, 2% slower on 96 states from HBM);
 //  * rows: 4 lines per 256-thread block in FP32, 2 lines per 128-thread block in FP64;
 //  * columns: 8 adjacent columns per block (64-byte row segments in FP32, 128 in FP64);
 //  * twiddles in LDS (FP32 always, FP64 columns when they fit) or read through the cache;
@@ -37,6 +39,7 @@ template <typename R, int N, bool STRIDED> struct PassTune {
     // keep LDS twiddles only if they do not cost lines per block
     static constexpr int TWM = (TW_PREF && fit(WANT, 1) == fit(WANT, 0)) ? 1 : 0;
     static constexpr int LPB = fit(WANT, TWM);
+    static constexpr int XS = WIDE ? 1 : 0;
 };
 
 // Small passes (lines up to 2048 points and few lines in the whole batch: one 256^2 state,
@@ -62,7 +65,7 @@ template <typename R, int N, bool STRIDED> struct SmallTune {
 
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0>
 using PassK = KCfg<R, N, PassTune<R, N, STRIDED>::E, PassTune<R, N, STRIDED>::LPB, STRIDED,
-                   PassTune<R, N, STRIDED>::TWM, DA, DB, GEN, 1, 0, 4, 5, OBS>;
+                   PassTune<R, N, STRIDED>::TWM, DA, DB, GEN, 1, 0, 4, 5, OBS, PassTune<R, N, STRIDED>::XS>;
 
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0>
 using PassKS = KCfg<R, N, SmallTune<R, N, STRIDED>::E, SmallTune<R, N, STRIDED>::LPB, STRIDED, 1, DA,

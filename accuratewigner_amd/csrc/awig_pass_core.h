@@ -170,7 +170,10 @@ template <bool STRIDED, int N, int LPB, int PADC = 4, int PADR = 5> struct LdsLa
 // stage-major LDS table: for the stage of span NS and radix RAD, entry (NS - E) + k*(RAD-1)
 // + (r-1) holds exp(-2 pi i r k / (NS RAD)), so a butterfly reads its RAD-1 twiddles at
 // consecutive addresses from one base.
-template <typename T2, int N, int E, int NS, int DIR, typename LAY, int TWM = 0>
+// XS 1: the exchange moves the real parts and then the imaginary parts through an LDS array
+// of reals (half the LDS of a complex exchange, twice the barriers); the butterfly outputs
+// wait in v meanwhile, so the register peak does not grow.
+template <typename T2, int N, int E, int NS, int DIR, typename LAY, int TWM = 0, int XS = 0>
 __device__ __forceinline__ void stockham(T2 *v, T2 *L, int l, int u, const T2 *__restrict__ tw) {
     constexpr int T = N / E;
     constexpr int RAD = (N / NS < E) ? (N / NS) : E;
@@ -193,7 +196,7 @@ __device__ __forceinline__ void stockham(T2 *v, T2 *L, int l, int u, const T2 *_
             }
         }
         dft_reg<DIR, RAD>(x);
-        if constexpr (LAST) {
+        if constexpr (LAST || XS) {
 #pragma unroll
             for (int r = 0; r < RAD; ++r) v[m + r * M] = x[r];
         } else {
@@ -202,19 +205,45 @@ __device__ __forceinline__ void stockham(T2 *v, T2 *L, int l, int u, const T2 *_
             for (int r = 0; r < RAD; ++r) L[LAY::at(base + r * NS, l)] = x[r];
         }
     }
-    if constexpr (!LAST) {
+    if constexpr (!LAST && !XS) {
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < E; ++kk) v[kk] = L[LAY::at(u + T * kk, l)];
         __syncthreads();
-        stockham<T2, N, E, NS * RAD, DIR, LAY, TWM>(v, L, l, u, tw);
+        stockham<T2, N, E, NS * RAD, DIR, LAY, TWM, XS>(v, L, l, u, tw);
+    } else if constexpr (!LAST) {
+        using R = decltype(v[0].x);
+        R *Lr = reinterpret_cast<R *>(L);
+        // real parts: outputs (b/NS)*NS*RAD + b%NS + r*NS of butterfly b = u + T*m sit in v[m + r*M]
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int b = u + T * m, k = b & (NS - 1), base = (b - k) * RAD + k;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) Lr[LAY::at(base + r * NS, l)] = v[m + r * M].x;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < E; ++kk) v[kk].x = Lr[LAY::at(u + T * kk, l)];
+        __syncthreads();
+        // imaginary parts (the .y of every v[m + r*M] is still the butterfly output)
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int b = u + T * m, k = b & (NS - 1), base = (b - k) * RAD + k;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) Lr[LAY::at(base + r * NS, l)] = v[m + r * M].y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < E; ++kk) v[kk].y = Lr[LAY::at(u + T * kk, l)];
+        __syncthreads();
+        stockham<T2, N, E, NS * RAD, DIR, LAY, TWM, XS>(v, L, l, u, tw);
     }
 }
 
-template <typename T2, int N, int E, typename LAY, int TWM>
+template <typename T2, int N, int E, typename LAY, int TWM, int XS = 0>
 __device__ __forceinline__ void fft_line(int dir, T2 *v, T2 *L, int l, int u, const T2 *tw) {
-    if (dir < 0) stockham<T2, N, E, 1, -1, LAY, TWM>(v, L, l, u, tw);
-    else stockham<T2, N, E, 1, +1, LAY, TWM>(v, L, l, u, tw);
+    if (dir < 0) stockham<T2, N, E, 1, -1, LAY, TWM, XS>(v, L, l, u, tw);
+    else stockham<T2, N, E, 1, +1, LAY, TWM, XS>(v, L, l, u, tw);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -228,18 +257,20 @@ __device__ __forceinline__ void fft_line(int dir, T2 *v, T2 *L, int l, int u, co
 // passes (GEN = 0) only multiply by line x element factor vectors (and an optional mask),
 // which keeps their code to two FFT bodies.
 template <typename R_, int N_, int E_, int LPB_, bool STRIDED_, int TWM_, int DA_ = 2, int DB_ = 2,
-          int GEN_ = 1, int MINW_ = 1, int PF_ = 0, int PADC_ = 4, int PADR_ = 5, int OBS_ = 0>
+          int GEN_ = 1, int MINW_ = 1, int PF_ = 0, int PADC_ = 4, int PADR_ = 5, int OBS_ = 0, int XS_ = 0>
 struct KCfg {
     using R = R_;
     static constexpr int N = N_, E = E_, LPB = LPB_, TWM = TWM_, DA = DA_, DB = DB_, GEN = GEN_;
     static constexpr int OBS = OBS_;     // 1: row passes also store weighted-sum block partials
+    static constexpr int XS = XS_;       // 1: LDS exchange in real and imaginary halves
     static constexpr int MINW = MINW_;   // minimum waves per SIMD the register budget must allow
     static constexpr int PF = PF_;       // 1: load the factor vectors with the data, before the first FFT
     static constexpr bool STRIDED = STRIDED_;
     static constexpr int T = N / E;
     static constexpr int THREADS = LPB * T;
     using LAY = LdsLayout<STRIDED, N, LPB, PADC_, PADR_>;
-    static constexpr int LDS_ELEMS = LAY::SIZE + (TWM != 0 ? N : 0);
+    static constexpr int XCH_ELEMS = XS ? (LAY::SIZE + 1) / 2 : LAY::SIZE;   // in complex units
+    static constexpr int LDS_ELEMS = XCH_ELEMS + (TWM != 0 ? N : 0);
 };
 
 template <typename C>
@@ -273,7 +304,7 @@ void k_line_pass(PassArgs a) {
     const T2 *tw = reinterpret_cast<const T2 *>(a.tw);
     if constexpr (C::TWM != 0) {
         // twiddle table into LDS; the first exchange barrier of any FFT precedes its first use
-        T2 *tl = smem + LAY::SIZE;
+        T2 *tl = smem + C::XCH_ELEMS;
         if ((C::DA == 2 ? a.dir_a != 0 : C::DA != 0) || (C::DB == 2 ? a.dir_b != 0 : C::DB != 0)) {
             if constexpr (C::TWM == 1) {
                 for (int q = tid; q < N; q += C::THREADS) tl[q] = tw[q];
@@ -325,9 +356,9 @@ void k_line_pass(PassArgs a) {
         }
     }
     if constexpr (C::DA == 2) {
-        if (a.dir_a) fft_line<T2, N, E, LAY, C::TWM>(a.dir_a, v, L, l_loc, u, tw);
+        if (a.dir_a) fft_line<T2, N, E, LAY, C::TWM, C::XS>(a.dir_a, v, L, l_loc, u, tw);
     } else if constexpr (C::DA != 0) {
-        stockham<T2, N, E, 1, C::DA, LAY, C::TWM>(v, L, l_loc, u, tw);
+        stockham<T2, N, E, 1, C::DA, LAY, C::TWM, C::XS>(v, L, l_loc, u, tw);
     }
 
     if (a.has_factor) {
@@ -376,9 +407,9 @@ void k_line_pass(PassArgs a) {
     }
 
     if constexpr (C::DB == 2) {
-        if (a.dir_b) fft_line<T2, N, E, LAY, C::TWM>(a.dir_b, v, L, l_loc, u, tw);
+        if (a.dir_b) fft_line<T2, N, E, LAY, C::TWM, C::XS>(a.dir_b, v, L, l_loc, u, tw);
     } else if constexpr (C::DB != 0) {
-        stockham<T2, N, E, 1, C::DB, LAY, C::TWM>(v, L, l_loc, u, tw);
+        stockham<T2, N, E, 1, C::DB, LAY, C::TWM, C::XS>(v, L, l_loc, u, tw);
     }
     if (C::GEN && a.chirp_out != 0.0) {
         // a second chirp after the second FFT (the twiddles of a two-level transform)
