@@ -13,7 +13,10 @@ constexpr int kLdsBudget = 160 * 1024;
 //    8 columns per 128 / 256-thread block; 14% faster at 1024^2 x 96 than E = 16), and
 //    exchange the real and the imaginary parts in turn (43 instead of 76 KB of LDS; 6% faster
 //    on a MALL-resident 24-state group of 1024^2, 2% slower on 96 states from HBM);
-//  * rows: 4 lines per 256-thread block in FP32, 2 lines per 128-thread block in FP64;
+//  * rows: 4 lines per 256-thread block in FP32, 2 lines per 128-thread block in FP64; FP32
+//    rows of 256 and of 1024+ points load their factor vectors with the data (PF, 4-7 %
+//    faster on MALL-resident batches; at 512 points it costs a wave per SIMD and 1.5 %; FP64
+//    rows go from 244 to 282 VGPRs and lose 40 %);
 //  * columns: 8 adjacent columns per block (64-byte row segments in FP32, 128 in FP64);
 //  * twiddles in LDS (FP32 always, FP64 columns when they fit) or read through the cache;
 //  * the largest power-of-two LPB that fits the 160 KB LDS.
@@ -40,6 +43,7 @@ template <typename R, int N, bool STRIDED> struct PassTune {
     static constexpr int TWM = (TW_PREF && fit(WANT, 1) == fit(WANT, 0)) ? 1 : 0;
     static constexpr int LPB = fit(WANT, TWM);
     static constexpr int XS = WIDE ? 1 : 0;
+    static constexpr int PF = (F32 && !STRIDED && (N == 256 || N >= 1024)) ? 1 : 0;
 };
 
 // Small passes (lines up to 2048 points and few lines in the whole batch: one 256^2 state,
@@ -65,7 +69,8 @@ template <typename R, int N, bool STRIDED> struct SmallTune {
 
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0>
 using PassK = KCfg<R, N, PassTune<R, N, STRIDED>::E, PassTune<R, N, STRIDED>::LPB, STRIDED,
-                   PassTune<R, N, STRIDED>::TWM, DA, DB, GEN, 1, 0, 4, 5, OBS, PassTune<R, N, STRIDED>::XS>;
+                   PassTune<R, N, STRIDED>::TWM, DA, DB, GEN, 1, PassTune<R, N, STRIDED>::PF, 4, 5, OBS,
+                   PassTune<R, N, STRIDED>::XS>;
 
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0>
 using PassKS = KCfg<R, N, SmallTune<R, N, STRIDED>::E, SmallTune<R, N, STRIDED>::LPB, STRIDED, 1, DA,
