@@ -26,7 +26,7 @@ and the pair reductions through it.
 """
 import numpy as np
 
-from . import wm_ops
+from . import graphs, wm_ops
 from .device import DeviceArray, empty, to_device
 from .engine import FWD, INV
 
@@ -75,6 +75,15 @@ class WignerMoyalDipoleControl:
         self._dots = empty((self.steps + 1, 4), np.float64)
         self._esz = np.dtype(s.state_dtype).itemsize
         self._state = None
+        # persistent inputs of one backward segment (fixed addresses, so that the segment's
+        # launches are captured once and replayed): V tables, the forward (+dt) and adjoint
+        # (-dt) unmerged factor vectors, and the pair-dot rows of the segment
+        K, p = self.checkpoint, s.plan
+        self._segV = (empty((K, Nx), np.float64), empty((K, Np), np.float64))
+        self._segF = {sg: (p.new_factors(Nx, 2 * K), p.new_factors(Np, 2 * K), p.new_factors(Np, K),
+                           p.new_factors(Nx, K)) for sg in (1, -1)}
+        self._segdots = empty((K + 1, 4), np.float64)
+        self._graphs = {}
 
     # -- helpers ---------------------------------------------------------------------------
     def _check_field(self, E):
@@ -96,6 +105,42 @@ class WignerMoyalDipoleControl:
         f = self.sys._factors_from_tables(n1 - n0, Vj.ptr, Vi.ptr, self._Kj.ptr, self._Ki.ptr,
                                           sign * self.dt, merged=merged, k_stride=0)
         return f, (Vj, Vi)
+
+    def _segment_factors(self, E, n0, n1):
+        """Fill the persistent V tables and factor vectors of steps n0..n1-1 (both signs)."""
+        e = E[self.piece[n0:n1]]
+        s, m = self.sys, n1 - n0
+        self._segV[0][:m] = self._V0j[None, :] + e[:, None] * s._kXX[None, :]
+        self._segV[1][:m] = self._V0i[None, :] + e[:, None] * s._kXXp[None, :]
+        for sg in (1, -1):
+            s._factors_from_tables(m, self._segV[0].ptr, self._segV[1].ptr, self._Kj.ptr, self._Ki.ptr,
+                                   sg * self.dt, merged=False, k_stride=0, dst=self._segF[sg])
+
+    def _segment_body(self, m, last):
+        """The launches of one backward segment of m steps from the persistent buffers:
+        recompute rho_{n0+1..n0+m} from the checkpoint in slot 0, then sweep lambda back
+        through the segment with the adjoint steps, with the pair dots of each step (and of
+        the segment's end state for the last segment) in the segment's dot rows."""
+        s, p = self.sys, self.sys.plan
+        esz = self._esz
+        Nx, Np = s.X_gridDIM, s.P_gridDIM
+        colL, colE, rowL, rowE = self._segF[1]
+        aL, aE, bL, bE = self._segF[-1]
+        for k in range(m):
+            # rho_{n0+k+1} from rho_{n0+k}: [EV, I0] out of place, [F1, EK, I1], [F0, EV]
+            p.run(self._slot(self._seg, k), 0, 0, INV, line_fac=colL.ptr + 2 * k * Nx * esz,
+                  elem_fac=colE.ptr + 2 * k * Np * esz, out=self._slot(self._seg, k + 1))
+            p.run(self._slot(self._seg, k + 1), 1, FWD, INV, line_fac=rowL.ptr + k * Np * esz,
+                  elem_fac=rowE.ptr + k * Nx * esz)
+            p.run(self._slot(self._seg, k + 1), 0, FWD, 0, line_fac=colL.ptr + (2 * k + 1) * Nx * esz,
+                  elem_fac=colE.ptr + (2 * k + 1) * Np * esz)
+        if last:
+            p.pair_dot(self._lam, self._slot(self._seg, m), self._u, self._v, self._segdots.ptr + m * 32)
+        for k in reversed(range(m)):
+            # lambda_{n0+k} = U^H lambda_{n0+k+1}: the step with conjugated factors
+            p.steps(self._lam, 1, aL.ptr + 2 * k * Nx * esz, aE.ptr + 2 * k * Np * esz,
+                    bL.ptr + k * Np * esz, bE.ptr + k * Nx * esz)
+            p.pair_dot(self._lam, self._slot(self._seg, k), self._u, self._v, self._segdots.ptr + k * 32)
 
     def _slot(self, arr, k):
         return arr.ptr + k * arr.shape[-2] * arr.shape[-1] * self._esz
@@ -137,31 +182,14 @@ class WignerMoyalDipoleControl:
         f = np.broadcast_to(((g.kX - J) * s.dXdP / norm)[None, :], (s.P_gridDIM, s.X_gridDIM))
         self._lam[:] = np.ascontiguousarray(f)
         wm_ops.rho2wigner_adjoint(p, g, self._lam)
-        esz = self._esz
-        Nx, Np = s.X_gridDIM, s.P_gridDIM
         for c in reversed(range(self._nck)):
             n0, n1 = c * K, min(self.steps, (c + 1) * K)
-            m = n1 - n0
+            m, last = n1 - n0, c == self._nck - 1
             self._seg.tensor[0].copy_(self._ckpt.tensor[c])
-            (colL, colE, rowL, rowE), keep = self._factors(E, n0, n1, 1.0, merged=False)
-            for k in range(m):
-                # rho_{n0+k+1} from rho_{n0+k}: [EV, I0] out of place, [F1, EK, I1], [F0, EV]
-                p.run(self._slot(self._seg, k), 0, 0, INV, line_fac=colL.ptr + 2 * k * Nx * esz,
-                      elem_fac=colE.ptr + 2 * k * Np * esz, out=self._slot(self._seg, k + 1))
-                p.run(self._slot(self._seg, k + 1), 1, FWD, INV, line_fac=rowL.ptr + k * Np * esz,
-                      elem_fac=rowE.ptr + k * Nx * esz)
-                p.run(self._slot(self._seg, k + 1), 0, FWD, 0, line_fac=colL.ptr + (2 * k + 1) * Nx * esz,
-                      elem_fac=colE.ptr + (2 * k + 1) * Np * esz)
-            (aL, aE, bL, bE), keep2 = self._factors(E, n0, n1, -1.0, merged=False)
-            if c == self._nck - 1:
-                p.pair_dot(self._lam, self._slot(self._seg, m), self._u, self._v,
-                           self._dots.ptr + n1 * 32)
-            for k in reversed(range(m)):
-                # lambda_{n0+k} = U^H lambda_{n0+k+1}: the step with conjugated factors
-                p.steps(self._lam, 1, aL.ptr + 2 * k * Nx * esz, aE.ptr + 2 * k * Np * esz,
-                        bL.ptr + k * Np * esz, bE.ptr + k * Nx * esz)
-                p.pair_dot(self._lam, self._slot(self._seg, k), self._u, self._v,
-                           self._dots.ptr + (n0 + k) * 32)
+            self._segment_factors(E, n0, n1)
+            graphs.run_chunk(self._graphs, (m, last), lambda m=m, last=last: self._segment_body(m, last), True)
+            rows = m + 1 if last else m
+            self._dots.tensor[n0:n0 + rows].copy_(self._segdots.tensor[:rows])
         d = self._dots.get()
         gb = 0.5 * self.dt * (d[:, 1] - d[:, 3])
         per_step = gb[:-1] + gb[1:]

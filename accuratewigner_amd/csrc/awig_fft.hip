@@ -125,6 +125,8 @@ constexpr int RED_THREADS = 256;
 constexpr int NSTAT = 8;
 
 __global__ void k_sum_partials(const double *partials, int nblk, int ncomp, double *out);
+__global__ void k_finish_sums(const double *partials, int nblk, int ncomp, double *out, double factor,
+                              double *dscale);
 
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
@@ -209,7 +211,7 @@ int launch_pair_dot(int prec, const void *a, const void *b, long long n_item, in
     else
         hipLaunchKernelGGL(k_pair_dot<float>, dim3(nblk, batch), dim3(RED_THREADS), 0, st, a, b, n_item, cols,
                            u, v, partials, nblk);
-    hipLaunchKernelGGL(k_sum_partials, dim3(batch), dim3(64), 0, st, partials, nblk, 4, out);
+    hipLaunchKernelGGL(k_finish_sums, dim3(batch), dim3(RED_THREADS), 0, st, partials, nblk, 4, out, 0.0, nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_pair_dot: %s", hipGetErrorString(e));
     return AW_OK;
@@ -386,7 +388,8 @@ int launch_stats(int prec, const void *data, long long n_item, int rows, int col
     else
         hipLaunchKernelGGL(k_stats<float>, dim3(nblk, batch), dim3(RED_THREADS), 0, st, data, n_item,
                            rows, cols, partials, nblk);
-    hipLaunchKernelGGL(k_sum_partials, dim3(batch), dim3(64), 0, st, partials, nblk, NSTAT, out);
+    hipLaunchKernelGGL(k_finish_sums, dim3(batch), dim3(RED_THREADS), 0, st, partials, nblk, NSTAT, out, 0.0,
+                       nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_stats: %s", hipGetErrorString(e));
     return AW_OK;
@@ -497,7 +500,12 @@ int launch_cyclic_obs(int prec, const void *data, int n, int batch, double *work
 
 int launch_sum_partials(const double *partials, int nblk, int ncomp, int batch, double *out,
                         hipStream_t st) {
-    hipLaunchKernelGGL(k_sum_partials, dim3(batch), dim3(64), 0, st, partials, nblk, ncomp, out);
+    // a whole block per item adds up to 8 components; more components, one thread each
+    if (ncomp <= 8)
+        hipLaunchKernelGGL(k_finish_sums, dim3(batch), dim3(RED_THREADS), 0, st, partials, nblk, ncomp, out, 0.0,
+                           nullptr);
+    else
+        hipLaunchKernelGGL(k_sum_partials, dim3(batch), dim3(64), 0, st, partials, nblk, ncomp, out);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_sum_partials: %s", hipGetErrorString(e));
     return AW_OK;

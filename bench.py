@@ -231,6 +231,57 @@ def main():
     gibbs["workload"] = ("config 2: Wigner-Bloch Gibbs state 512x512, beta 0 -> 2 in 400 steps of 0.005, "
                          "<H>(beta) and purity every step (5 timed solves)")
 
+    # -- config 3: strong-field von Neumann rho(x, x'), 1024^2, x in [-200, 200), soft-core V,
+    #    E(t) = 0.05 sin^2(pi t / T) cos(0.06 t) over 4 cycles (20944 steps of 0.02), cos^(1/8)
+    #    absorbing edge, Ehrenfest lists (dipole acceleration -<V'>) every step; FP64 and mixed --
+    from accuratewigner_amd import RhoVNeumannCUDA1D
+    from accuratewigner_amd.strong_field import rv_ground_state_rho, rv_sys_params
+    p3 = rv_sys_params()
+    steps3 = int(round(p3["T_pulse"] / p3["dt"]))
+    rho0 = rv_ground_state_rho(p3, nsteps=5000)
+    strong, accel = {}, {}
+    for prec in ("fp64", "mixed"):
+        g3 = RhoVNeumannCUDA1D(precision=prec, **p3)
+        g3.set_rho(rho0)
+        g3.propagate(16)                  # compiles and captures outside the timed run
+        g3.set_rho(rho0)
+        g3.t = 0.
+        for name in ("X_average", "P_average", "X_average_RHS", "P_average_RHS", "hamiltonian_average"):
+            getattr(g3, name).clear()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g3.propagate(steps3)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        accel[prec] = np.array(g3.P_average_RHS)
+        strong[prec] = {"s": el, "us_per_step": el / steps3 * 1e6, "gp_steps_per_s": 1024 ** 2 * steps3 / el}
+        del g3
+    strong["mixed"]["accel_err_of_max"] = float(np.max(np.abs(accel["mixed"] - accel["fp64"]))
+                                                / np.max(np.abs(accel["fp64"])))
+    strong["workload"] = ("config 3: von Neumann rho 1024x1024, 4-cycle pulse, %d steps of 0.02, "
+                          "Ehrenfest lists every step (1 run each)" % steps3)
+
+    # -- config 4: adjoint gradient d<x>(T)/dE_k, WM 512^2, x, p in [-6, 6), V = x^4/4 - x^2/2,
+    #    64 field pieces ~ U(-0.1, 0.1), 2000 steps of 0.005, checkpoints every 50 steps --
+    from accuratewigner_amd.adjoint import WignerMoyalDipoleControl
+    c4 = WignerMoyalDipoleControl(X_gridDIM=512, X_amplitude=6., P_gridDIM=512, P_amplitude=6., dt=0.005,
+                                  steps=2000, n_pieces=64, V0="0.25 * X * X * X * X - 0.5 * X * X",
+                                  W0="exp(-2. * (X + 1.) * (X + 1.) - 0.5 * P * P) / M_PI", checkpoint=50)
+    E4 = np.random.default_rng(4).uniform(-0.1, 0.1, 64)
+    c4.gradient(E4)
+    times = []
+    for r in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        c4.forward(E4)
+        c4.gradient()
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    adjoint = {"workload": "config 4: forward + adjoint gradient, WM FP64 512x512, 2000 steps, 64 pieces, "
+                           "checkpoint 50 (3 timed runs)",
+               "p50_ms": float(np.percentile(times, 50)), "max_ms": float(np.max(times))}
+    del c4
+
     serving = serving_run(args.serve_jobs) if args.serve_jobs > 0 else None
 
     traffic = None
@@ -274,6 +325,8 @@ def main():
                                              "p50_ms": float(np.percentile(lat_nr, 50)),
                                              "p99_ms": float(np.percentile(lat_nr, 99))}},
         "gibbs": gibbs,
+        "strong_field": strong,
+        "adjoint": adjoint,
         "serving": serving,
         "cpu_baseline": cpu,
     }
