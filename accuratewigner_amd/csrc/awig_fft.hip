@@ -420,12 +420,15 @@ int launch_cyclic_diag(int prec, const void *data, int n, int batch, double *wor
     return AW_OK;
 }
 
-// Per-step Ehrenfest sums of square items from the row-block partials of k_cyclic_diag:
-// thread d forms c_d (fixed order over the row blocks) and adds Re(c_d pw_hat[q][d]) for the
-// momentum weights, and Re rho_dd (times xw[q][d]) for the trace and the coordinate weights.
-// Block partials go to `partials`; the last block of the item (ticket counter, reset for the
-// next launch) adds them in block order into out[b][0 .. 1 + nx + np), divides the momentum
-// terms by n, and writes the normalisation scale 1 / (out[b][0] factor) when dscale is given.
+// Per-step Ehrenfest sums of square items from the row-block partials of k_cyclic_diag.
+// A block covers 64 diagonals d; its 4 wavefronts each add a quarter of the nrb row-block
+// partials of every d (fixed order), and the quarters are added in order through LDS.  Then
+// for each d: Re(c_d pw_hat[q][d]) for the momentum weights, and Re rho_dd (times xw[q][d])
+// for the trace and the coordinate weights.  Block partials go to `partials`; the last
+// block of the item (ticket counter, reset for the next launch; n/64 blocks, so the
+// serialised fences stay short) adds them in block order into out[b][0 .. 1 + nx + np),
+// divides the momentum terms by n, and writes the normalisation scale 1 / (out[b][0] factor)
+// when dscale is given.
 template <typename R>
 __global__ __launch_bounds__(256) void k_cyclic_obs(const void *data, int n, const double *work, int nrb,
                                                     const double *xw, int nx, const double2 *pw_hat, int np,
@@ -433,14 +436,31 @@ __global__ __launch_bounds__(256) void k_cyclic_obs(const void *data, int n, con
                                                     long long out_bstride, double factor, double *dscale) {
     using T2 = typename Cx<R>::T;
     const int b = blockIdx.y;
-    const int d = blockIdx.x * 256 + threadIdx.x;
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (d < n) {
-        const double *w = work + (long long)b * nrb * 2 * n + 2 * d;
+    const int dl = threadIdx.x & 63, quarter = threadIdx.x >> 6;
+    const int d = blockIdx.x * 64 + dl;
+    __shared__ double2 csum[4][64];
+    {
         double cr = 0.0, ci = 0.0;
-        for (int r = 0; r < nrb; ++r) {
-            cr += w[(long long)r * 2 * n];
-            ci += w[(long long)r * 2 * n + 1];
+        if (d < n) {
+            const double2 *w = reinterpret_cast<const double2 *>(work + (long long)b * nrb * 2 * n) + d;
+            const int r0 = quarter * (nrb / 4), r1 = (quarter == 3) ? nrb : r0 + nrb / 4;
+#pragma unroll 4
+            for (int r = r0; r < r1; ++r) {
+                const double2 v = w[(long long)r * n];
+                cr += v.x;
+                ci += v.y;
+            }
+        }
+        csum[quarter][dl] = make_double2(cr, ci);
+    }
+    __syncthreads();
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (quarter == 0 && d < n) {
+        double cr = 0.0, ci = 0.0;
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+            cr += csum[qq][dl].x;
+            ci += csum[qq][dl].y;
         }
         for (int q = 0; q < np; ++q) {
             const double2 h = pw_hat[(long long)q * n + d];
@@ -485,7 +505,7 @@ int launch_cyclic_obs(int prec, const void *data, int n, int batch, double *work
         hipLaunchKernelGGL(k_cyclic_diag<double>, grid, dim3(256), 0, st, data, n, work);
     else
         hipLaunchKernelGGL(k_cyclic_diag<float>, grid, dim3(256), 0, st, data, n, work);
-    const dim3 g2((n + 255) / 256, batch);
+    const dim3 g2((n + 63) / 64, batch);
     const double2 *ph = reinterpret_cast<const double2 *>(pw_hat);
     if (prec == AW_PREC_F64)
         hipLaunchKernelGGL(k_cyclic_obs<double>, g2, dim3(256), 0, st, data, n, work, n / kDiagRowsPerBlock, xw, nx,
