@@ -28,6 +28,12 @@ def test_library_exports_every_declared_symbol():
     assert set(names) <= set(_lib.exported_symbols())
 
 
+def test_every_declared_function_has_ctypes_argument_types():
+    """A declared entry point without argtypes would pass pointers as C ints."""
+    missing = [n for n in _header_functions() if n not in _lib._SIGNATURES and n != "aw_last_error"]
+    assert not missing, missing
+
+
 def test_status_codes_raise_mapped_exceptions():
     if not os.path.exists(_lib.LIB_PATH):
         pytest.skip("libawig.so not built")
