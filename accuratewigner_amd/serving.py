@@ -127,6 +127,8 @@ class _Pool:
                            to_device(self.g.kPPp)]
         self.jobs = []
         self.remaining = []
+        self.pending = []        # (job, pinned host W, event) of results still being copied out
+        self._free_host = []     # pinned host buffers to reuse
 
     @property
     def slot_bytes(self):
@@ -206,9 +208,11 @@ class _Pool:
         return n
 
     def finish(self, clock):
-        """Finalise slots whose steps are done: W = rho2wigner, W /= Re sum W dXdP, copy out."""
+        """Finalise slots whose steps are done: enqueue W = rho2wigner, W /= Re sum W dXdP and
+        the copy of W to pinned host memory on the pool's stream, then compact the slots.  The
+        host reads the result once the copy's event has completed (``collect``), so it keeps
+        launching the other pools' work meanwhile."""
         import torch
-        done = []
         k = len(self.jobs) - 1
         while k >= 0:
             if self.remaining[k] == 0:
@@ -219,15 +223,15 @@ class _Pool:
                     st = self.plan1.stats(self.W1)
                     self.plan1.scale_from_stats(st.ptr, 8, 0, 0, g.dXdP, self.dscale)
                     self.plan1.scale(self.W1, self.dscale)
-                    W = self.W1.get()
-                job.summary = dict(norm=float(np.sum(W.real) * g.dXdP),
-                                   x_mean=float(np.sum(W.real * g.kX[None, :]) * g.dXdP),
-                                   p_mean=float(np.sum(W.real * g.kP[:, None]) * g.dXdP))
-                if job.keep_result:
-                    job.result = W
-                job.t_done = clock()
+                    host = (self._free_host.pop() if self._free_host
+                            else torch.empty(self.W1.tensor.shape, dtype=self.W1.tensor.dtype, pin_memory=True))
+                    host.copy_(self.W1.tensor, non_blocking=True)
+                    ev = torch.cuda.Event()
+                    ev.record(self.stream)
+                self.pending.append((job, host, ev))
                 last = len(self.jobs) - 1
                 if k != last:
+                    # stream order keeps these copies after the transform read slot k
                     with torch.cuda.stream(self.stream):
                         self.state.tensor[k].copy_(self.state.tensor[last])
                         for t in (self.colL, self.colE, self.rowL, self.rowE):
@@ -236,8 +240,28 @@ class _Pool:
                     self.remaining[k] = self.remaining[last]
                 self.jobs.pop()
                 self.remaining.pop()
-                done.append(job)
             k -= 1
+
+    def collect(self, clock, block=False):
+        """Jobs whose result has reached host memory (all pending ones with ``block``)."""
+        done, keep = [], []
+        for job, host, ev in self.pending:
+            if block:
+                ev.synchronize()
+            if not (block or ev.query()):
+                keep.append((job, host, ev))
+                continue
+            W = host.numpy()
+            g = self.g
+            job.summary = dict(norm=float(np.sum(W.real) * g.dXdP),
+                               x_mean=float(np.sum(W.real * g.kX[None, :]) * g.dXdP),
+                               p_mean=float(np.sum(W.real * g.kP[:, None]) * g.dXdP))
+            if job.keep_result:
+                job.result = W.copy()
+            job.t_done = clock()
+            self._free_host.append(host)
+            done.append(job)
+        self.pending = keep
         return done
 
 
@@ -276,7 +300,7 @@ class Server:
         incoming = collections.deque(sorted(jobs, key=lambda j: j.arrival))
         waiting = collections.deque()
         finished = []
-        while incoming or waiting or any(p.jobs for p in self.pools.values()):
+        while incoming or waiting or any(p.jobs or p.pending for p in self.pools.values()):
             now = clock()
             while incoming and (incoming[0].arrival <= now or not realtime):
                 waiting.append(incoming.popleft())
@@ -289,15 +313,17 @@ class Server:
                 else:
                     waiting.append(job)
             active = [p for p in self.pools.values() if p.jobs]
-            if not active:
-                if incoming:
-                    time.sleep(max(0.0, incoming[0].arrival - clock()))
-                continue
             for p in active:
                 self.stats["gp_steps"] += p.tick(self.chunk) * len(p.jobs) * p.N * p.N
                 self.stats["rounds"] += 1
             for p in active:
-                finished.extend(p.finish(clock))
+                p.finish(clock)
+            # results that reached the host; wait for them only when nothing else can run
+            idle = not any(p.jobs for p in self.pools.values()) and not waiting
+            for p in self.pools.values():
+                finished.extend(p.collect(clock, block=idle and not incoming))
+            if idle and incoming and not any(p.pending for p in self.pools.values()):
+                time.sleep(max(0.0, incoming[0].arrival - clock()))
         self.stats["wall_s"] = clock()
         return finished
 
