@@ -1,0 +1,129 @@
+"""The reduction entry points of the C ABI against NumPy on random data (FP64 and mixed):
+signed cyclic diagonal sums, weighted sums (with sum |z|^2 and the normalisation scale),
+their block partials finished by aw_finish_sums, the partials a row step pass stores as it
+writes its output, the von Neumann Ehrenfest sums, and the adjoint's pair dots."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def _tol(prec):
+    return 1e-12 if prec == "fp64" else 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_cyclic_diag_sums(prec):
+    from accuratewigner_amd.device import empty, to_device
+    from accuratewigner_amd.engine import Plan
+    n, B = 128, 2
+    p = Plan(n, n, B, prec)
+    z = _rand(np.random.default_rng(1), (B, n, n)).astype(p.state_dtype)
+    out = empty((B, n), np.complex128)
+    p.cyclic_diag_sums(to_device(z), out)
+    i = np.arange(n)
+    ref = np.array([[(-1) ** d * np.sum(z[b, i, (i + d) % n]) for d in range(n)] for b in range(B)])
+    assert np.max(np.abs(out.get() - ref)) < _tol(prec) * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_weighted_sums_partials_and_finish(prec):
+    from accuratewigner_amd.device import empty, to_device
+    from accuratewigner_amd.engine import Plan, finish_sums
+    rows, cols, B, nw = 64, 256, 3, 5
+    rng = np.random.default_rng(2)
+    p = Plan(rows, cols, B, prec)
+    z = _rand(rng, (B, rows, cols)).astype(p.state_dtype)
+    w = _rand(rng, (nw, rows, cols))
+    zd, wd = to_device(z), to_device(w)
+    out = empty((B, nw + 1), np.float64)
+    ds = empty((B, 2), np.float64)
+    p.weighted_sums(zd, wd, nw, out, factor=0.25, dscale=ds, norm2=True)
+    zz = z.astype(np.complex128)
+    ref = np.concatenate([np.einsum("qij,bij->bq", np.conj(w), zz).real,
+                          np.sum(np.abs(zz) ** 2, axis=(1, 2))[:, None]], axis=1)
+    got = out.get()
+    assert np.max(np.abs(got - ref)) < _tol(prec) * np.max(np.abs(ref))
+    assert np.allclose(ds.get()[:, 0], 1.0 / (got[:, 0] * 0.25), rtol=1e-14, atol=0)
+    assert np.all(ds.get()[:, 1] == 0)
+    # the same sums from the block partials and one finish launch: bit-identical
+    nblk = p.weighted_partials(None, None, nw, norm2=True)
+    part = empty((B, nblk, nw + 1), np.float64)
+    assert p.weighted_partials(zd, wd, nw, part, norm2=True) == nblk
+    out2 = empty((B, nw + 1), np.float64)
+    finish_sums(part, nblk, nw + 1, B, out2)
+    assert np.array_equal(out2.get(), got)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("shape", [(64, 256), (128, 1024)])
+def test_row_pass_stores_weighted_partials_of_its_output(prec, shape):
+    from accuratewigner_amd.device import empty, to_device
+    from accuratewigner_amd.engine import FWD, INV, Plan, finish_sums
+    rows, cols = shape
+    B, nw = 2, 3
+    rng = np.random.default_rng(3)
+    p = Plan(rows, cols, B, prec)
+    z = _rand(rng, (B, rows, cols)).astype(p.state_dtype)
+    w = to_device(_rand(rng, (nw, rows, cols)))
+    lf = to_device((np.exp(1j * rng.random(rows)) / cols).astype(p.state_dtype))
+    ef = to_device(np.exp(1j * rng.random(cols)).astype(p.state_dtype))
+    a, b = to_device(z), to_device(z)
+    nblk = p.row_obs_blocks()
+    part = empty((B, nblk, nw), np.float64)
+    p.run(a, 1, FWD, INV, line_fac=lf, elem_fac=ef, obs_weights=w, obs_nw=nw, obs_partials=part)
+    p.run(b, 1, FWD, INV, line_fac=lf, elem_fac=ef)
+    assert np.array_equal(a.get(), b.get())          # the epilogue leaves the output alone
+    got = empty((B, nw), np.float64)
+    finish_sums(part, nblk, nw, B, got)
+    ref = empty((B, nw), np.float64)
+    p.weighted_sums(b, w, nw, ref)
+    r = ref.get()
+    assert np.max(np.abs(got.get() - r)) < 1e-12 * np.max(np.abs(r))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_cyclic_diag_obs(prec):
+    from accuratewigner_amd.device import empty, to_device
+    from accuratewigner_amd.engine import Plan
+    n = 256
+    rng = np.random.default_rng(4)
+    p = Plan(n, n, 1, prec)
+    z = _rand(rng, (1, n, n)).astype(p.state_dtype)
+    xw = rng.standard_normal((3, n))
+    pw = rng.standard_normal((2, n))
+    pw_hat = np.fft.fft(pw, axis=1)
+    out = empty((1, 8), np.float64)
+    ds = empty((1, 2), np.float64)
+    p.cyclic_diag_obs(to_device(z), to_device(xw), 3, to_device(pw_hat), 2, out, 8, factor=0.5, dscale=ds)
+    zz = z[0].astype(np.complex128)
+    i = np.arange(n)
+    c = np.array([(-1) ** d * np.sum(zz[i, (i + d) % n]) for d in range(n)])
+    diag = zz[i, i].real
+    ref = np.concatenate([[diag.sum()], xw @ diag, (pw @ np.fft.fft(c)).real / n])
+    got = out.get()[0, :6]
+    assert np.max(np.abs(got - ref)) < _tol(prec) * np.max(np.abs(ref))
+    assert abs(ds.get()[0, 0] - 1.0 / (got[0] * 0.5)) < 1e-14 * abs(ds.get()[0, 0])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_pair_dot(prec):
+    from accuratewigner_amd.device import empty, to_device
+    from accuratewigner_amd.engine import Plan
+    rows, cols, B = 64, 128, 2
+    rng = np.random.default_rng(5)
+    p = Plan(rows, cols, B, prec)
+    a = _rand(rng, (B, rows, cols)).astype(p.state_dtype)
+    b = _rand(rng, (B, rows, cols)).astype(p.state_dtype)
+    u, v = rng.standard_normal(cols), rng.standard_normal(rows)
+    out = empty((B, 4), np.float64)
+    p.pair_dot(to_device(a), to_device(b), to_device(u), to_device(v), out)
+    ab = np.conj(a.astype(np.complex128)) * b.astype(np.complex128)
+    su = np.einsum("bij,j->b", ab, u)
+    sv = np.einsum("bij,i->b", ab, v)
+    ref = np.stack([su.real, su.imag, sv.real, sv.imag], axis=1)
+    assert np.max(np.abs(out.get() - ref)) < _tol(prec) * np.max(np.abs(ref))
