@@ -13,6 +13,10 @@ constexpr int kLdsBudget = 160 * 1024;
 //    8 columns per 128 / 256-thread block; 14% faster at 1024^2 x 96 than E = 16), and
 //    exchange the real and the imaginary parts in turn (43 instead of 76 KB of LDS; 6% faster
 //    on a MALL-resident 24-state group of 1024^2, 2% slower on 96 states from HBM);
+//  * FP32 rows of E = 16 exchange the same way: a 1024-point row block then takes 25 instead
+//    of 42 KB of LDS and 4 instead of 3 waves run per SIMD (110 VGPRs; 1.2 % faster on
+//    1024^2 x 96, neutral at 512^2).  FP64 passes keep the complex exchange: with split
+//    halves they reach 3 waves only by spilling, and measured no faster;
 //  * rows: 4 lines per 256-thread block in FP32, 2 lines per 128-thread block in FP64; FP32
 //    rows of 256 and of 1024+ points load their factor vectors with the data (PF, 4-7 %
 //    faster on MALL-resident batches; at 512 points it costs a wave per SIMD and 1.5 %; FP64
@@ -42,7 +46,7 @@ template <typename R, int N, bool STRIDED> struct PassTune {
     // keep LDS twiddles only if they do not cost lines per block
     static constexpr int TWM = (TW_PREF && fit(WANT, 1) == fit(WANT, 0)) ? 1 : 0;
     static constexpr int LPB = fit(WANT, TWM);
-    static constexpr int XS = WIDE ? 1 : 0;
+    static constexpr int XS = (WIDE || (F32 && !STRIDED && E == 16)) ? 1 : 0;
     static constexpr int PF = (F32 && !STRIDED && (N == 256 || N >= 1024)) ? 1 : 0;
 };
 

@@ -406,10 +406,19 @@ void k_line_pass(PassArgs a) {
         }
     }
 
+    // The second FFT would reuse the first one's LDS addresses (exchange reads and writes
+    // and twiddle reads: a VGPR each per element, ~80 at E = 32) and keep them live through
+    // the factor multiply.  Hiding the lane's line and position from the compiler here makes
+    // it recompute them, a few integer instructions per FFT.  FP32 1024-point columns go
+    // from 234 to 156 VGPRs (2 -> 3 waves per SIMD, 1.46e11 -> 1.51e11 grid-point steps/s
+    // on 1024^2 x 96).  FP64 passes are held at 2 waves by their LDS exchange and only pay
+    // the recomputation (1.8 % on 1024^2 x 48), so they keep the addresses.
+    int l_b = l_loc, u_b = u;
+    if constexpr (C::DA != 0 && C::DB != 0 && sizeof(R) == 4) __asm__ volatile("" : "+v"(l_b), "+v"(u_b));
     if constexpr (C::DB == 2) {
-        if (a.dir_b) fft_line<T2, N, E, LAY, C::TWM, C::XS>(a.dir_b, v, L, l_loc, u, tw);
+        if (a.dir_b) fft_line<T2, N, E, LAY, C::TWM, C::XS>(a.dir_b, v, L, l_b, u_b, tw);
     } else if constexpr (C::DB != 0) {
-        stockham<T2, N, E, 1, C::DB, LAY, C::TWM, C::XS>(v, L, l_loc, u, tw);
+        stockham<T2, N, E, 1, C::DB, LAY, C::TWM, C::XS>(v, L, l_b, u_b, tw);
     }
     if (C::GEN && a.chirp_out != 0.0) {
         // a second chirp after the second FFT (the twiddles of a two-level transform)

@@ -12,6 +12,7 @@ A "step" is one Strang step of the whole batch.  value = B * 1024 * 1024 * steps
 Prints ONE JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
+import contextlib
 import json
 import os
 import sys
@@ -143,8 +144,7 @@ def main():
     N, B, K, W = args.grid, args.batch, args.steps, args.warmup
     if args.copy_only:
         gbps, ms = copy_bandwidth(args.copy_only, reps=3)
-        print(json.dumps({"copy_MiB": args.copy_only, "copy_GBps": gbps, "ms": ms}), flush=True)
-        return
+        return {"copy_MiB": args.copy_only, "copy_GBps": gbps, "ms": ms}
     copy_gbps, _ = copy_bandwidth()
 
     # -- headline: mixed precision, batch B -------------------------------------------------
@@ -330,8 +330,12 @@ def main():
         "serving": serving,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 if __name__ == "__main__":
-    main()
+    # the propagator constructors print a memory report (as the reference's do): send it and
+    # everything else to stderr, so that stdout carries the one JSON line
+    with contextlib.redirect_stdout(sys.stderr):
+        result = main()
+    print(json.dumps(result), flush=True)
