@@ -137,7 +137,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the propagators shard over independent states; see DESIGN.md)", file=sys.stderr)
+              "(the propagators would shard over independent states, one batch slice per GPU)", file=sys.stderr)
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
