@@ -482,9 +482,14 @@ __global__ __launch_bounds__(256) void k_cyclic_obs(const void *data, int n, con
     if (!last) return;
     __threadfence();
     const int ncomp = 1 + nx + np;
+    // all block partials at once (one load per thread, they were just written by blocks on
+    // other XCDs), then the sums in block order from LDS
+    __shared__ double fin[kCycObsMaxBlocks * 8];
+    for (int i = threadIdx.x; i < nblk * 8; i += blockDim.x) fin[i] = pb[i];
+    __syncthreads();
     if (threadIdx.x < ncomp) {
         double s = 0.0;
-        for (int k = 0; k < nblk; ++k) s += pb[(long long)k * 8 + threadIdx.x];
+        for (int k = 0; k < nblk; ++k) s += fin[k * 8 + threadIdx.x];
         if (threadIdx.x > nx) s /= n;
         out[(long long)b * out_bstride + threadIdx.x] = s;
         if (threadIdx.x == 0 && dscale) {
@@ -500,6 +505,8 @@ __global__ __launch_bounds__(256) void k_cyclic_obs(const void *data, int n, con
 int launch_cyclic_obs(int prec, const void *data, int n, int batch, double *work, const double *xw, int nx,
                       const double *pw_hat, int np, double *partials, unsigned *counters, double *out,
                       long long out_bstride, double factor, double *dscale, hipStream_t st) {
+    if ((n + 63) / 64 > kCycObsMaxBlocks)
+        return set_error(AW_ERR_INVALID_SIZE, "cyclic observables: n = %d above %d", n, 64 * kCycObsMaxBlocks);
     const dim3 grid((n + 255) / 256, n / kDiagRowsPerBlock, batch);
     if (prec == AW_PREC_F64)
         hipLaunchKernelGGL(k_cyclic_diag<double>, grid, dim3(256), 0, st, data, n, work);

@@ -58,6 +58,8 @@ int launch_stats(int prec, const void *data, long long n_item, int rows, int col
 int launch_pair_dot(int prec, const void *a, const void *b, long long n_item, int cols, const double *u,
                     const double *v, int batch, double *partials, int nblk, double *out, hipStream_t st);
 constexpr int kDiagRowsPerBlock = 16;
+// blocks of k_cyclic_obs per item (64 diagonals each): lines up to 4096 points
+constexpr int kCycObsMaxBlocks = 64;
 int launch_cyclic_diag(int prec, const void *data, int n, int batch, double *work, double *out, hipStream_t st);
 constexpr int kWsumMaxBlocks = 512;
 int launch_weighted_sums(int prec, const void *data, long long n_item, const double *w, int nw, int norm2,

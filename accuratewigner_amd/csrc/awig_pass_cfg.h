@@ -65,10 +65,14 @@ template <typename R, int N> constexpr long long small_lines_max() {
     return (N <= 256 || (sizeof(R) == 4 && N <= 512)) ? 2 * kSmallLines : kSmallLines;
 }
 
+// Columns of 1024 points go 4 to a block (32 / 64-byte row segments): the von Neumann step
+// on one 1024^2 state (three passes) takes 64.3 instead of 68.1 us in FP64 and 45.8 instead
+// of 49.6 us in mixed precision.  At 256 and 512 points, and with 1 or 8 columns at 1024,
+// it is no faster or slower; E = 4 or 16 at 1024 points measured no faster either.
 template <typename R, int N, bool STRIDED> struct SmallTune {
     static constexpr int E = (N / 128 < 4) ? 4 : ((N / 128 > 16) ? 16 : N / 128);
     static constexpr int T = N / E;
-    static constexpr int LPB = STRIDED ? 2 : (T >= 64 ? 1 : 64 / T);
+    static constexpr int LPB = STRIDED ? (N == 1024 ? 4 : 2) : (T >= 64 ? 1 : 64 / T);
 };
 
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0>
