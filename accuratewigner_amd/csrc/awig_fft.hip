@@ -439,12 +439,29 @@ __global__ __launch_bounds__(256) void k_cyclic_obs(const void *data, int n, con
     const int dl = threadIdx.x & 63, quarter = threadIdx.x >> 6;
     const int d = blockIdx.x * 64 + dl;
     __shared__ double2 csum[4][64];
+    // the loads that do not depend on the row-block partials first, so that their latency
+    // overlaps that of the partials
+    double2 h[7];
+    double xq[7], rd = 0.0;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        h[q] = make_double2(0.0, 0.0);
+        xq[q] = 0.0;
+    }
+    if (quarter == 0 && d < n) {
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            if (q < np) h[q] = pw_hat[(long long)q * n + d];
+            if (q < nx) xq[q] = xw[(long long)q * n + d];
+        }
+        rd = reinterpret_cast<const T2 *>(data)[(long long)b * n * n + (long long)d * n + d].x;
+    }
     {
         double cr = 0.0, ci = 0.0;
         if (d < n) {
             const double2 *w = reinterpret_cast<const double2 *>(work + (long long)b * nrb * 2 * n) + d;
             const int r0 = quarter * (nrb / 4), r1 = (quarter == 3) ? nrb : r0 + nrb / 4;
-#pragma unroll 4
+#pragma unroll 16
             for (int r = r0; r < r1; ++r) {
                 const double2 v = w[(long long)r * n];
                 cr += v.x;
@@ -462,14 +479,13 @@ __global__ __launch_bounds__(256) void k_cyclic_obs(const void *data, int n, con
             cr += csum[qq][dl].x;
             ci += csum[qq][dl].y;
         }
-        for (int q = 0; q < np; ++q) {
-            const double2 h = pw_hat[(long long)q * n + d];
-            acc[1 + nx + q] = cr * h.x - ci * h.y;
-        }
-        const T2 z = reinterpret_cast<const T2 *>(data)[(long long)b * n * n + (long long)d * n + d];
-        const double rd = z.x;
+#pragma unroll
+        for (int q = 0; q < 7; ++q)
+            if (q < np) acc[1 + nx + q] = cr * h[q].x - ci * h[q].y;
         acc[0] = rd;
-        for (int q = 0; q < nx; ++q) acc[1 + q] = rd * xw[(long long)q * n + d];
+#pragma unroll
+        for (int q = 0; q < 7; ++q)
+            if (q < nx) acc[1 + q] = rd * xq[q];
     }
     const int nblk = gridDim.x;
     double *pb = partials + (long long)b * nblk * 8;

@@ -65,6 +65,8 @@ template <typename R, int N> constexpr long long small_lines_max() {
     return (N <= 256 || (sizeof(R) == 4 && N <= 512)) ? 2 * kSmallLines : kSmallLines;
 }
 
+// The step kinds load their factor vectors and mask with the data (PF = 2), so that their
+// latency overlaps the data loads instead of following the first FFT.
 // Columns of 1024 points go 4 to a block (32 / 64-byte row segments): the von Neumann step
 // on one 1024^2 state (three passes) takes 64.3 instead of 68.1 us in FP64 and 45.8 instead
 // of 49.6 us in mixed precision.  At 256 and 512 points, and with 1 or 8 columns at 1024,
@@ -73,6 +75,7 @@ template <typename R, int N, bool STRIDED> struct SmallTune {
     static constexpr int E = (N / 128 < 4) ? 4 : ((N / 128 > 16) ? 16 : N / 128);
     static constexpr int T = N / E;
     static constexpr int LPB = STRIDED ? (N == 1024 ? 4 : 2) : (T >= 64 ? 1 : 64 / T);
+    static constexpr int PF = 2;
 };
 
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0>
@@ -82,7 +85,7 @@ using PassK = KCfg<R, N, PassTune<R, N, STRIDED>::E, PassTune<R, N, STRIDED>::LP
 
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0>
 using PassKS = KCfg<R, N, SmallTune<R, N, STRIDED>::E, SmallTune<R, N, STRIDED>::LPB, STRIDED, 1, DA,
-                    DB, GEN, 1, 0, 4, 5, OBS>;
+                    DB, GEN, 1, GEN ? 0 : SmallTune<R, N, STRIDED>::PF, 4, 5, OBS>;
 
 template <typename C>
 int launch_cfg(const PassArgs &a, hipStream_t st) {

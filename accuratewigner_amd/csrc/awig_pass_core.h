@@ -264,7 +264,8 @@ struct KCfg {
     static constexpr int OBS = OBS_;     // 1: row passes also store weighted-sum block partials
     static constexpr int XS = XS_;       // 1: LDS exchange in real and imaginary halves
     static constexpr int MINW = MINW_;   // minimum waves per SIMD the register budget must allow
-    static constexpr int PF = PF_;       // 1: load the factor vectors with the data, before the first FFT
+    static constexpr int PF = PF_;       // 1: load the factor vectors with the data, before the first FFT;
+                                         // 2: the mask as well
     static constexpr bool STRIDED = STRIDED_;
     static constexpr int T = N / E;
     static constexpr int THREADS = LPB * T;
@@ -336,6 +337,7 @@ void k_line_pass(PassArgs a) {
     // optional prefetch of the element factors: their load latency then overlaps the data
     // loads instead of stalling between the two FFTs (costs 2E registers)
     T2 efp[C::PF ? E : 1];
+    R mkp[C::PF >= 2 ? E : 1];
     T2 lfp = mk<T2>(1.0, 0.0);
     if constexpr (C::PF) {
         if (a.has_factor && a.elem_fac) {
@@ -344,6 +346,13 @@ void k_line_pass(PassArgs a) {
             for (int k = 0; k < E; ++k) efp[k] = ef[T * k];
         }
         if (a.has_factor && a.line_fac) lfp = reinterpret_cast<const T2 *>(a.line_fac)[b * a.lf_bstride + line];
+        if constexpr (C::PF >= 2) {
+            if (a.has_factor && a.mask) {
+                const R *mk2 = reinterpret_cast<const R *>(a.mask) + b * a.m_bstride + off0;
+#pragma unroll
+                for (int k = 0; k < E; ++k) mkp[k] = mk2[k * kstep];
+            }
+        }
     }
 
     if (C::GEN && a.sign_in) {
@@ -384,7 +393,7 @@ void k_line_pass(PassArgs a) {
         if (a.mask) {
             const R *mk2 = reinterpret_cast<const R *>(a.mask) + b * a.m_bstride + off0;
 #pragma unroll
-            for (int k = 0; k < E; ++k) v[k] = cscale(v[k], mk2[k * kstep]);
+            for (int k = 0; k < E; ++k) v[k] = cscale(v[k], C::PF >= 2 ? mkp[k] : mk2[k * kstep]);
         }
         if (C::GEN && a.ctab) {
             const T2 *ct = reinterpret_cast<const T2 *>(a.ctab) + b * a.c_bstride + off0;
