@@ -83,6 +83,7 @@ class aw_pass(ctypes.Structure):
         ("scale", ctypes.c_double), ("dscale", ctypes.c_void_p), ("out", ctypes.c_void_p),
         ("ctab", ctypes.c_void_p), ("ctab_bstride", ctypes.c_longlong), ("chirp_out", ctypes.c_double),
         ("obs_weights", ctypes.c_void_p), ("obs_nw", ctypes.c_int), ("obs_partials", ctypes.c_void_p),
+        ("obs_mode", ctypes.c_int), ("obs_norm2", ctypes.c_int), ("obs_norm2_weights", ctypes.c_void_p),
     ]
 
 
@@ -110,6 +111,7 @@ _SIGNATURES = {
     "aw_fft_axis": [_vp, _vp, _vp, _i, _i, _vp],
     "aw_pass_run": [_vp, _vp, ctypes.POINTER(aw_pass), _vp],
     "aw_row_obs_blocks": [_vp, ctypes.POINTER(_i)],
+    "aw_pass_obs_blocks": [_vp, _i, ctypes.POINTER(_i)],
     "aw_step_sequence": [_vp, _vp, ctypes.POINTER(aw_steps), _vp],
     "aw_step_group": [_vp, _i, _i, ctypes.POINTER(_i)],
     "aw_make_factors": [_vp, _vp, _ll, _i, _i, _i, _vp, _ll, _d, _vp, _ll, _d, _d, _vp, _ll, _i,

@@ -174,9 +174,20 @@ static int pass_run_n(aw_plan *p, void *data, const aw_pass *ps, int nbatch, aw_
     if (ps->obs_weights) {
         if (ps->obs_nw < 1 || ps->obs_nw > 8 || !ps->obs_partials)
             return set_error(AW_ERR_INVALID_VALUE, "obs_nw must be 1..8 with an obs_partials buffer");
+        if (ps->obs_mode != 0 && ps->obs_mode != 1)
+            return set_error(AW_ERR_INVALID_VALUE, "obs_mode must be 0 or 1 (got %d)", ps->obs_mode);
+        if (ps->obs_norm2 != 0 && ps->obs_norm2 != 1)
+            return set_error(AW_ERR_INVALID_VALUE, "obs_norm2 must be 0 or 1 (got %d)", ps->obs_norm2);
+        if (ps->obs_nw + ps->obs_norm2 > 8)
+            return set_error(AW_ERR_INVALID_VALUE, "obs_nw + obs_norm2 must be at most 8");
+        if (ps->obs_norm2_weights && !ps->obs_norm2)
+            return set_error(AW_ERR_INVALID_VALUE, "obs_norm2_weights without obs_norm2");
         a.obs_w = static_cast<const double2 *>(ps->obs_weights);
         a.obs_nw = ps->obs_nw;
         a.obs_partials = ps->obs_partials;
+        a.obs_mode = ps->obs_mode;
+        a.obs_norm2 = ps->obs_norm2;
+        a.obs_n2w = ps->obs_norm2_weights;
     }
     a.has_factor = (ps->line_fac || ps->elem_fac || ps->mask || ps->ctab || ps->chirp != 0.0 ||
                     ps->scale != 1.0 || ps->dscale) ? 1 : 0;
@@ -184,12 +195,19 @@ static int pass_run_n(aw_plan *p, void *data, const aw_pass *ps, int nbatch, aw_
     return launch_pass(p->prec, n, strided, a, (hipStream_t)stream);
 }
 
-int aw_row_obs_blocks(const aw_plan *p, int *nblk) {
+int aw_row_obs_blocks(const aw_plan *p, int *nblk) { return aw_pass_obs_blocks(p, 1, nblk); }
+
+int aw_pass_obs_blocks(const aw_plan *p, int axis, int *nblk) {
     if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
     if (!nblk) return set_error(AW_ERR_INVALID_VALUE, "NULL nblk");
-    const int lpb = row_obs_lpb(p->prec, p->cols, (long long)p->rows * p->batch);
-    if (lpb < 1) return set_error(AW_ERR_INVALID_SIZE, "unsupported row length %d", p->cols);
-    *nblk = p->rows / lpb;
+    if (axis != 0 && axis != 1) return set_error(AW_ERR_INVALID_VALUE, "axis must be 0 or 1");
+    const bool strided = axis == 0;
+    const int n = strided ? p->rows : p->cols, nl = strided ? p->cols : p->rows;
+    const int lpb = obs_lpb(p->prec, n, strided, (long long)nl * p->batch);
+    if (lpb < 1 || nl % lpb)
+        return set_error(AW_ERR_INVALID_SIZE, "no weighted-sum partials for %s of %d points over %d lines",
+                         strided ? "columns" : "rows", n, nl);
+    *nblk = nl / lpb;
     return AW_OK;
 }
 

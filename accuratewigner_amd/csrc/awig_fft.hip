@@ -44,11 +44,11 @@ int set_error(int code, const char *fmt, ...) {
 // ---------------------------------------------------------------------------------------
 int launch_pass_f32(int n, bool strided, int kind, const PassArgs &a, hipStream_t st);
 int launch_pass_f64(int n, bool strided, int kind, const PassArgs &a, hipStream_t st);
-int row_obs_lpb_f32(int n, long long total_lines);
-int row_obs_lpb_f64(int n, long long total_lines);
+int obs_lpb_f32(int n, bool strided, long long total_lines);
+int obs_lpb_f64(int n, bool strided, long long total_lines);
 
-int row_obs_lpb(int prec, int n, long long total_lines) {
-    return prec == AW_PREC_F64 ? row_obs_lpb_f64(n, total_lines) : row_obs_lpb_f32(n, total_lines);
+int obs_lpb(int prec, int n, bool strided, long long total_lines) {
+    return prec == AW_PREC_F64 ? obs_lpb_f64(n, strided, total_lines) : obs_lpb_f32(n, strided, total_lines);
 }
 
 int launch_pass(int prec, int n, bool strided, const PassArgs &a, hipStream_t st) {
@@ -61,9 +61,17 @@ int launch_pass(int prec, int n, bool strided, const PassArgs &a, hipStream_t st
         else if (a.dir_a == AW_FORWARD && a.dir_b == 0) kind = 3;
     }
     if (a.obs_w) {
-        if (kind != 2 || strided)
-            return set_error(AW_ERR_INVALID_VALUE, "weighted-sum partials need a plain [forward, factor, inverse] row pass");
-        kind = 4;
+        if (a.obs_mode == 0) {
+            if (kind != 2 || strided)
+                return set_error(AW_ERR_INVALID_VALUE,
+                                 "partials of the output need a plain [forward, factor, inverse] row pass");
+            kind = 4;
+        } else {
+            if (!strided || (kind != 2 && kind != 3))
+                return set_error(AW_ERR_INVALID_VALUE,
+                                 "partials after the factor need a plain [forward, factor(, inverse)] column pass");
+            kind = kind == 2 ? 5 : 6;
+        }
     }
     return prec == AW_PREC_F64 ? launch_pass_f64(n, strided, kind, a, st)
                                : launch_pass_f32(n, strided, kind, a, st);

@@ -33,9 +33,13 @@ struct PassArgs {
     const void *tw;         // N roots exp(-2 pi i t / N), plan precision
     const void *ctab; long long c_bstride;   // complex per-element factor table (general passes)
     double chirp_out;                        // exp(i chirp_out lcoord ecoord) after the second FFT
-    const double2 *obs_w;   // row passes: complex128 weights [obs_nw][rows][cols] shared by the items
+    const double2 *obs_w;   // complex128 weights [obs_nw][rows][cols] shared by the items
     int obs_nw;             // 1..8
-    double *obs_partials;   // [batch][nlines / LPB][obs_nw]: Re sum conj(w) out over each block
+    double *obs_partials;   // [batch][nlines / LPB][obs_nw + obs_norm2]: block partials
+    int obs_norm2;          // 1: one more component, sum |v|^2 (times obs_n2w when given)
+    const double *obs_n2w;  // real weights [rows][cols] of that component, or NULL
+    int obs_mode;           // 0: partials of the pass output (row passes); 1: of the values
+                            // right after the factor multiply (column passes)
 };
 
 struct FactorArgs {
@@ -51,7 +55,7 @@ struct FactorArgs {
 
 int launch_pass(int prec, int n, bool strided, const PassArgs &a, hipStream_t st);
 // lines per block of the row step pass with weighted-sum partials (axis 1, line length n)
-int row_obs_lpb(int prec, int n, long long total_lines);
+int obs_lpb(int prec, int n, bool strided, long long total_lines);
 int launch_make_factors(int prec, const FactorArgs &f, hipStream_t st);
 int launch_stats(int prec, const void *data, long long n_item, int rows, int cols, int batch,
                  double *partials, int nblk, double *out, hipStream_t st);

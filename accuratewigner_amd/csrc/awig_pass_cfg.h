@@ -101,7 +101,11 @@ int launch_cfg(const PassArgs &a, hipStream_t st) {
 
 // kind: 0 general (run-time directions, signs, chirps, diagonal output);
 //       1 step [factor, inverse FFT]; 2 step [forward, factor, inverse]; 3 step [forward, factor];
-//       4 row step [forward, factor, inverse] that also stores weighted-sum block partials
+//       4 row step [forward, factor, inverse] that also stores weighted-sum block partials of
+//         its output;
+//       5 / 6 column step [forward, factor, inverse] / [forward, factor] that stores the block
+//         partials of the values right after the factor multiply
+// (the partial kinds need blocks of whole waves: obs_lpb_n reports 0 where they are not)
 template <typename R, int N, bool STRIDED>
 int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
     if constexpr (N <= 2048) {
@@ -110,6 +114,14 @@ int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
             case 4:
                 if constexpr (!STRIDED) return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0, 1>>(a, st);
                 return set_error(AW_ERR_INVALID_VALUE, "weighted-sum partials need a row pass");
+            case 5:
+            case 6:
+                if constexpr (STRIDED && PassKS<R, N, STRIDED, -1, 1, 0>::THREADS % 64 == 0) {
+                    if (kind == 5) return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0, 2>>(a, st);
+                    return launch_cfg<PassKS<R, N, STRIDED, -1, 0, 0, 2>>(a, st);
+                }
+                return set_error(AW_ERR_INVALID_VALUE,
+                                 "partials after the factor: column passes of whole waves only");
             case 1: return launch_cfg<PassKS<R, N, STRIDED, 0, 1, 0>>(a, st);
             case 2: return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0>>(a, st);
             case 3: return launch_cfg<PassKS<R, N, STRIDED, -1, 0, 0>>(a, st);
@@ -121,6 +133,13 @@ int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
     case 4:
         if constexpr (!STRIDED) return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0, 1>>(a, st);
         return set_error(AW_ERR_INVALID_VALUE, "weighted-sum partials need a row pass");
+    case 5:
+    case 6:
+        if constexpr (STRIDED && PassK<R, N, STRIDED, -1, 1, 0>::THREADS % 64 == 0) {
+            if (kind == 5) return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0, 2>>(a, st);
+            return launch_cfg<PassK<R, N, STRIDED, -1, 0, 0, 2>>(a, st);
+        }
+        return set_error(AW_ERR_INVALID_VALUE, "partials after the factor: column passes of whole waves only");
     case 1: return launch_cfg<PassK<R, N, STRIDED, 0, 1, 0>>(a, st);
     case 2: return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0>>(a, st);
     case 3: return launch_cfg<PassK<R, N, STRIDED, -1, 0, 0>>(a, st);
@@ -128,25 +147,27 @@ int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
     }
 }
 
-// lines per block of the row pass that launch_kind_n picks for total_lines lines of n points
-template <typename R, int N> long long row_lpb_n(long long total_lines) {
+// lines per block of the pass that launch_kind_n picks for total_lines lines of n points,
+// or 0 where its blocks are not whole waves (no weighted-sum partials there)
+template <typename R, int N, bool STRIDED> int obs_lpb_n(long long total_lines) {
     if constexpr (N <= 2048) {
-        if (total_lines <= small_lines_max<R, N>()) return SmallTune<R, N, false>::LPB;
+        if (total_lines <= small_lines_max<R, N>()) {
+            using C = PassKS<R, N, STRIDED, -1, 1, 0>;
+            return C::THREADS % 64 == 0 ? C::LPB : 0;
+        }
     }
-    return PassTune<R, N, false>::LPB;
+    using C = PassK<R, N, STRIDED, -1, 1, 0>;
+    return C::THREADS % 64 == 0 ? C::LPB : 0;
 }
 
-template <typename R> int row_lpb(int n, long long total_lines) {
+template <typename R> int obs_lpb(int n, bool strided, long long total_lines) {
+#define AW_CASE(NN) \
+    case NN: return strided ? obs_lpb_n<R, NN, true>(total_lines) : obs_lpb_n<R, NN, false>(total_lines);
     switch (n) {
-    case 64: return int(row_lpb_n<R, 64>(total_lines));
-    case 128: return int(row_lpb_n<R, 128>(total_lines));
-    case 256: return int(row_lpb_n<R, 256>(total_lines));
-    case 512: return int(row_lpb_n<R, 512>(total_lines));
-    case 1024: return int(row_lpb_n<R, 1024>(total_lines));
-    case 2048: return int(row_lpb_n<R, 2048>(total_lines));
-    case 4096: return int(row_lpb_n<R, 4096>(total_lines));
+        AW_CASE(64) AW_CASE(128) AW_CASE(256) AW_CASE(512) AW_CASE(1024) AW_CASE(2048) AW_CASE(4096)
     default: return 0;
     }
+#undef AW_CASE
 }
 
 template <typename R>

@@ -261,7 +261,8 @@ template <typename R_, int N_, int E_, int LPB_, bool STRIDED_, int TWM_, int DA
 struct KCfg {
     using R = R_;
     static constexpr int N = N_, E = E_, LPB = LPB_, TWM = TWM_, DA = DA_, DB = DB_, GEN = GEN_;
-    static constexpr int OBS = OBS_;     // 1: row passes also store weighted-sum block partials
+    static constexpr int OBS = OBS_;     // weighted-sum block partials of the output (1) or of the
+                                         // values right after the factor multiply (2)
     static constexpr int XS = XS_;       // 1: LDS exchange in real and imaginary halves
     static constexpr int MINW = MINW_;   // minimum waves per SIMD the register budget must allow
     static constexpr int PF = PF_;       // 1: load the factor vectors with the data, before the first FFT;
@@ -273,6 +274,56 @@ struct KCfg {
     static constexpr int XCH_ELEMS = XS ? (LAY::SIZE + 1) / 2 : LAY::SIZE;   // in complex units
     static constexpr int LDS_ELEMS = XCH_ELEMS + (TWM != 0 ? N : 0);
 };
+
+// Block partials of the weighted sums of a pass's line values v (all lines of the block
+// belong to one item):  Re sum conj(w_q) v for q < obs_nw, then (obs_norm2) sum |v|^2, with
+// real weights obs_n2w when given, into obs_partials[item][block][component].  Each thread
+// adds its elements, then the waves and then the waves of the block are added in order.
+template <typename C, typename T2>
+__device__ __forceinline__ void obs_block_partials(const PassArgs &a, const T2 *v, bool active, int off0,
+                                                   int kstep, long long blk, int nl, int tid) {
+    constexpr int N = C::N, E = C::E, LPB = C::LPB;
+    static_assert(C::THREADS % 64 == 0, "weighted-sum partials of whole waves");
+    const int nw = a.obs_nw, nc = a.obs_nw + a.obs_norm2;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (active) {
+        const long long nit = (long long)N * nl;
+        const double2 *w = a.obs_w + off0;   // the weights are shared by the items
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const double x = v[k].x, y = v[k].y;
+            for (int q = 0; q < nw; ++q) {
+                const double2 c = w[q * nit + (long long)k * kstep];
+                acc[q] += c.x * x + c.y * y;
+            }
+            if (a.obs_norm2) {
+                double m2 = x * x + y * y;
+                if (a.obs_n2w) m2 *= a.obs_n2w[off0 + (long long)k * kstep];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (q == nw) acc[q] += m2;
+            }
+        }
+    }
+    __shared__ double ored[C::THREADS / 64][8];
+    const int lane = tid & 63, wid = tid >> 6;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        double s = acc[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if (lane == 0) ored[wid][q] = s;
+    }
+    __syncthreads();
+    if (tid < nc) {
+        double s = 0.0;
+        for (int q = 0; q < C::THREADS / 64; ++q) s += ored[q][tid];
+        const long long g0 = blk * LPB;
+        const long long bb = g0 / nl;
+        const long long bi = (g0 - bb * nl) / LPB;
+        a.obs_partials[(bb * (nl / LPB) + bi) * nc + tid] = s;
+    }
+}
 
 template <typename C>
 __global__ __launch_bounds__(C::THREADS) __attribute__((amdgpu_waves_per_eu(C::MINW)))
@@ -415,6 +466,7 @@ void k_line_pass(PassArgs a) {
         }
     }
 
+    if constexpr (C::OBS == 2) obs_block_partials<C>(a, v, active, off0, kstep, blk, nl, tid);
     // The second FFT would reuse the first one's LDS addresses (exchange reads and writes
     // and twiddle reads: a VGPR each per element, ~80 at E = 32) and keep them live through
     // the factor multiply.  Hiding the lane's line and position from the compiler here makes
@@ -452,42 +504,7 @@ void k_line_pass(PassArgs a) {
             if (par & 1) v[k] = cscale(v[k], R(-1));
         }
     }
-    if constexpr (C::OBS) {
-        // block partials of Re sum conj(w_q) out over this block's lines (all of one item):
-        // each thread its elements, then the waves, then the waves of the block
-        static_assert(!STRIDED && C::THREADS % 64 == 0, "row passes of whole waves");
-        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (active) {
-            const long long nit = (long long)N * nl;
-            const double2 *w = a.obs_w + off0;   // the weights are shared by the items
-#pragma unroll
-            for (int k = 0; k < E; ++k) {
-                const double x = v[k].x, y = v[k].y;
-                for (int q = 0; q < a.obs_nw; ++q) {
-                    const double2 c = w[q * nit + (long long)k * kstep];
-                    acc[q] += c.x * x + c.y * y;
-                }
-            }
-        }
-        __shared__ double ored[C::THREADS / 64][8];
-        const int lane = tid & 63, wid = tid >> 6;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            double s = acc[q];
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-            if (lane == 0) ored[wid][q] = s;
-        }
-        __syncthreads();
-        if (tid < a.obs_nw) {
-            double s = 0.0;
-            for (int q = 0; q < C::THREADS / 64; ++q) s += ored[q][tid];
-            const long long g0 = blk * LPB;
-            const long long bb = g0 / nl;
-            const long long bi = (g0 - bb * nl) / LPB;
-            a.obs_partials[(bb * (nl / LPB) + bi) * a.obs_nw + tid] = s;
-        }
-    }
+    if constexpr (C::OBS == 1) obs_block_partials<C>(a, v, active, off0, kstep, blk, nl, tid);
     if (!active) return;
     if (!C::GEN || a.out_mode == 0) {
         T2 *dst = reinterpret_cast<T2 *>(a.out) + item + off0;

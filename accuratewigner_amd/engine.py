@@ -75,7 +75,8 @@ class Plan:
             line_bstride=0, elem_bstride=0, mask_bstride=0, sign_in=0, sign_out=0,
             chirp=0.0, lstep=0.0, lcenter=0.0, estep=0.0, ecenter=0.0, lfft=0, efft=0,
             scale=1.0, dscale=None, out=None, out_mode=0, stream=None, ctab=None, ctab_bstride=0,
-            chirp_out=0.0, obs_weights=None, obs_nw=0, obs_partials=None):
+            chirp_out=0.0, obs_weights=None, obs_nw=0, obs_partials=None, obs_mode=0, obs_norm2=0,
+            obs_norm2_weights=None):
         ps = _lib.aw_pass()
         ps.axis, ps.dir_a, ps.dir_b = axis, dir_a, dir_b
         ps.sign_in, ps.sign_out, ps.out_mode = sign_in, sign_out, out_mode
@@ -89,6 +90,7 @@ class Plan:
         ps.out = _p(out)
         ps.ctab, ps.ctab_bstride, ps.chirp_out = _p(ctab), ctab_bstride, chirp_out
         ps.obs_weights, ps.obs_nw, ps.obs_partials = _p(obs_weights), obs_nw, _p(obs_partials)
+        ps.obs_mode, ps.obs_norm2, ps.obs_norm2_weights = obs_mode, int(obs_norm2), _p(obs_norm2_weights)
         _lib.call("aw_pass_run", self.handle, ctypes.c_void_p(_p(data)), ctypes.byref(ps),
                   ctypes.c_void_p(stream if stream is not None else current_stream()))
 
@@ -170,6 +172,13 @@ class Plan:
         nb = ctypes.c_int()
         _lib.call("aw_row_obs_blocks", self.handle, ctypes.byref(nb))
         return nb.value
+
+    def obs_blocks(self, axis):
+        """Blocks per item of the weighted-sum partials of a step pass along ``axis``, or 0
+        where that pass stores none (blocks of fewer than 64 threads)."""
+        nb = ctypes.c_int()
+        rc = _lib.load().aw_pass_obs_blocks(self.handle, int(axis), ctypes.byref(nb))
+        return nb.value if rc == 0 else 0
 
     def weighted_partials(self, data, weights, nw, partials=None, norm2=False):
         """Block partials [batch, nblk, nw (+1)] of weighted_sums into ``partials`` (a device

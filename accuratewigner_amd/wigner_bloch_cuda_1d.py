@@ -12,7 +12,7 @@ import numpy as np
 
 from . import graphs
 from .device import empty, to_device
-from .engine import FWD, INV
+from .engine import FWD, INV, finish_sums
 from .wigner_moyal_cuda_1d import WignerMoyalCUDA1D
 
 CHECK_CHUNK = 32
@@ -49,6 +49,7 @@ class WignerBlochCUDA1D(WignerMoyalCUDA1D):
         self.cuda_consts += "#define K_min %.15e \n" % self.K_min
         self._bloch_cache = {}
         self._gibbs_w = {}
+        self._gibbs_mw = {}        # weights and factors of the merged Gibbs chunks (or None)
         self._gibbs_bufs = {}      # persistent per-chunk sums and captured chunks of the Gibbs loop
         self.gibbs_stopped_at = None
 
@@ -168,6 +169,73 @@ class WignerBlochCUDA1D(WignerMoyalCUDA1D):
         self.plan.weighted_sums(self.rho, g, nw, out_ptr, self.dXdP, self._dscale, norm2=True)
         self._scale_pending = True
 
+    def _gibbs_merge(self, dbeta, record):
+        """Weights and factors of merged Gibbs chunks, or None where they do not apply.
+
+        Inside a chunk the closing column pass of step s, [F0, C1], and the opening one of
+        step s + 1, [C0, I0], run as one pass [F0, C0 C1, I0]; its values right after the
+        factor are v = C0 rho_s, with rho_s the state after step s (C0, C1 real).  So the sums
+        of step s are Re <g / C0, v> and sum |v|^2 / C0^2, taken in that pass (aw_pass
+        obs_mode 1).  The state is normalised once per chunk (by the last step's norm): the
+        purity and the recorded averages are ratios of the sums, and the Bloch factors are at
+        most 1, so the unnormalised state only decays within a chunk.  Where C0 is below
+        1e-150 (FP64) or 1e-17 (mixed; the merged factor C0^2 / Np then stays a normal FP32
+        number above it) the weights are 0: there |rho_s| <= C0 / Np |F0 Y| is below that
+        fraction of the state (the potential on the X' grid grows far past V(X_amplitude),
+        and exp(-dbeta V / 4) underflows to 0 at its edges)."""
+        key = (float(dbeta), tuple(record), float(self.t))
+        if key in self._gibbs_mw:
+            return self._gibbs_mw[key]
+        mw = None
+        p = self.plan
+        nblk = p.obs_blocks(0)
+        if nblk:
+            Nx, Np = self.X_gridDIM, self.P_gridDIM
+            q = -0.25 * float(dbeta)
+            vx = self._V0_x.get().reshape(-1)[:Nx]
+            vxp = self._V0_xp.get().reshape(-1)[:Np]
+            # C0 in FP64, as the opening factor vectors of _bloch_factors (v = 0)
+            cl = np.exp(q * (vx - self.V_min_x)) * (1.0 - 2.0 * (np.arange(Nx) & 1))
+            ce = np.exp(q * (vxp - self.V_min_xp)) * (1.0 - 2.0 * (np.arange(Np) & 1))
+            c0 = np.outer(ce, cl)
+            keep = np.abs(c0) > (1e-150 if self.precision == "fp64" else 1e-17)
+            inv = np.where(keep, 1.0 / np.where(keep, c0, 1.0), 0.0)
+            g = self._gibbs_weights(record)
+            colLm, colEm = p.new_factors(Nx, 1), p.new_factors(Np, 1)
+            p.factors(colLm, Nx, 1, A=self._V0_x.ptr, ca=2 * q, c0=-2 * q * self.V_min_x, mode=1)
+            p.factors(colEm, Np, 1, A=self._V0_xp.ptr, ca=2 * q, c0=-2 * q * self.V_min_xp, mode=1,
+                      scale=1.0 / Np)
+            mw = dict(nblk=nblk, nc=len(record) + 2, g=g, colLm=colLm, colEm=colEm,
+                      g_mid=to_device(g.get() * inv[None]), n2w=to_device(inv * inv))
+        self._gibbs_mw[key] = mw
+        return mw
+
+    def _gibbs_merged_chunk(self, dbeta, n, mw, fb):
+        """n Bloch steps in 2n + 1 passes; the per-step sums [Re sum W, recorded sums,
+        sum |rho|^2] of the unnormalised states go to fb["stats"][:n], and the last step's
+        normalisation scale to self._dscale (applied by the next pass that opens a step)."""
+        colL, colE, rowL, rowE = self._bloch_factors(dbeta)
+        csz = np.dtype(self.state_dtype).itemsize
+        Nx, Np, B = self.X_gridDIM, self.P_gridDIM, self.batch
+        p, rho = self.plan, self.rho
+        nblk, nc = mw["nblk"], mw["nc"]
+        ring = fb["ring"]
+        stride = B * nblk * nc * 8
+        p.run(rho, 0, 0, INV, line_fac=colL.ptr, elem_fac=colE.ptr, dscale=self._dscale)
+        for s in range(n):
+            p.run(rho, 1, FWD, INV, line_fac=rowL.ptr, elem_fac=rowE.ptr)
+            part = ring.ptr + s * stride
+            if s < n - 1:
+                p.run(rho, 0, FWD, INV, line_fac=mw["colLm"].ptr, elem_fac=mw["colEm"].ptr,
+                      obs_weights=mw["g_mid"], obs_nw=nc - 1, obs_partials=part, obs_mode=1, obs_norm2=1,
+                      obs_norm2_weights=mw["n2w"])
+            else:
+                p.run(rho, 0, FWD, 0, line_fac=colL.ptr + Nx * csz, elem_fac=colE.ptr + Np * csz,
+                      obs_weights=mw["g"], obs_nw=nc - 1, obs_partials=part, obs_mode=1, obs_norm2=1)
+        finish_sums(ring, nblk, nc, n * B, fb["stats"])
+        finish_sums(ring.ptr + (n - 1) * stride, nblk, nc, B, fb["last"], self.dXdP, self._dscale)
+        self._scale_pending = True
+
     def get_gibbs_state(self, kT=None, dbeta=None, record=None):
         """wigner_bloch_cuda_1d.py:119-155: infinite-temperature start, n Bloch steps with a
         purity < 1 check after each (stop with a warning otherwise).  Steps run in chunks with
@@ -189,6 +257,7 @@ class WignerBlochCUDA1D(WignerMoyalCUDA1D):
         fast = self.obs_by_adjoint and m + 2 <= 8
         g = self._gibbs_weights(record) if fast else None
         nc = m + 2 if fast else 8         # per-step values per item in `stats`
+        mw = None                         # merged chunks (fast path only)
         if fast:
             # every step applies the pending scale (1 at the start), so that the chunks are
             # alike and a full chunk can be captured once and replayed (fixed addresses)
@@ -202,7 +271,11 @@ class WignerBlochCUDA1D(WignerMoyalCUDA1D):
                 # the first call allocates the plan's scratch (never inside a capture)
                 self.plan.weighted_sums(self.rho, g, m + 1, fb["stats"], norm2=True)
             self._bloch_factors(float(dbeta))      # cached factor vectors, made outside any capture
-            gkey = (float(dbeta), tuple(record), float(self.t))
+            mw = self._gibbs_merge(dbeta, record)  # likewise
+            if mw is not None and "ring" not in fb:
+                fb["ring"] = empty((CHECK_CHUNK, self.batch, mw["nblk"], nc), np.float64)
+                fb["last"] = empty((self.batch, nc), np.float64)
+            gkey = (float(dbeta), tuple(record), float(self.t), mw is not None)
         rec = []
         k = 0
         while k < num_beta_steps:
@@ -222,11 +295,17 @@ class WignerBlochCUDA1D(WignerMoyalCUDA1D):
                 if record:
                     self._obs_launch(record, self.wignerfunction, float(self.t), ring.ptr + s * self.batch * m * 8)
 
+            def steps(n_):
+                if fast and mw is not None:
+                    self._gibbs_merged_chunk(dbeta, n_, mw, fb)
+                else:
+                    for s in range(n_):
+                        step(s)
+
             if fast:
-                graphs.run_chunk(fb["graphs"], gkey, lambda n=n: [step(s) for s in range(n)], n == CHECK_CHUNK)
+                graphs.run_chunk(fb["graphs"], gkey, lambda n=n: steps(n), n == CHECK_CHUNK)
             else:
-                for s in range(n):
-                    step(s)
+                steps(n)
             st = stats.get()[:n]
             pur = (self._purities(st, n, nc, m + 1, float(self.P_gridDIM)) if fast
                    else self._purities(st, n))
@@ -237,8 +316,7 @@ class WignerBlochCUDA1D(WignerMoyalCUDA1D):
                 self.rho[:] = saved
                 self._scale_pending = saved_scale
                 self._dscale[:] = saved_ds
-                for s in range(stop + 1):
-                    step(s)
+                steps(stop + 1)
                 self._finish_gibbs(fast)
                 self.gibbs_stopped_at = k + stop
                 print("Warning: Gibbs state calculations interupted because purity = %.10f > 1."

@@ -97,17 +97,30 @@ typedef struct aw_pass {
     void *out;
     const void *ctab; long long ctab_bstride;         /* complex [rows][cols] factor table */
     double chirp_out;
-    /* plain [forward, factor, inverse] row passes only: block partials of the weighted sums
-     * Re sum conj(obs_weights[q]) out (q < obs_nw <= 8; complex128 [obs_nw][rows][cols],
-     * shared by the items) into obs_partials[b][k][q], k < aw_row_obs_blocks(), to be added
-     * by aw_finish_sums.  NULL weights: none. */
+    /* Block partials of the weighted sums Re sum conj(obs_weights[q]) v (q < obs_nw <= 8;
+     * complex128 [obs_nw][rows][cols], shared by the items) into obs_partials[b][k][c],
+     * k < aw_pass_obs_blocks(), c < obs_nw + obs_norm2, to be added by aw_finish_sums.
+     *   obs_mode 0: v is the pass output; plain [forward, factor, inverse] row passes.
+     *   obs_mode 1: v is the line right after the factor multiply; plain [forward, factor,
+     *               inverse] or [forward, factor] column passes.  With the factor of a
+     *               merged pass F = C0 * C1 (C1 closing one step, C0 opening the next) and
+     *               weights w / conj(C0), these are the sums of the state between the steps
+     *               (wigner_bloch_cuda_1d.py:139-147 takes them after every step).
+     * obs_norm2 = 1 adds the component sum |v|^2 (times obs_norm2_weights, real
+     * [rows][cols], when given).  NULL weights: none. */
     const void *obs_weights; int obs_nw;
     double *obs_partials;
+    int obs_mode, obs_norm2;
+    const double *obs_norm2_weights;
 } aw_pass;
 
 int aw_pass_run(aw_plan *plan, void *data, const aw_pass *pass, aw_stream stream);
 
-/* Blocks per item (k range) of the obs_partials of a row pass over the plan's batch. */
+/* Blocks per item (k range) of the obs_partials of a pass along axis (0: columns, 1: rows)
+ * over the plan's batch; AW_ERR_INVALID_SIZE where that pass has no partials (blocks of
+ * fewer than 64 threads: columns of 64 points in single-state passes). */
+int aw_pass_obs_blocks(const aw_plan *plan, int axis, int *nblk);
+/* The same for rows (axis 1). */
 int aw_row_obs_blocks(const aw_plan *plan, int *nblk);
 
 /*

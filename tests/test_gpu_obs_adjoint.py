@@ -103,3 +103,29 @@ def test_gibbs_purity_guard_with_sums_from_rho():
     assert abs(np.sum(Wa.real) * a.dXdP - 1.) < 1e-12
     assert np.linalg.norm(Wa - Wb) < 1e-12 * np.linalg.norm(Wb)
     assert 0.5 <= a.get_purity() < 0.51
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_gibbs_merged_chunks_match_the_per_step_sums(prec, monkeypatch):
+    """Merged Gibbs chunks (2 passes a step, sums taken right after the factor of the merged
+    column pass, one normalisation per chunk) against the same loop with 3 passes and one
+    weighted-sum launch a step: 400 steps, i.e. captured full chunks and an eager partial one."""
+    from accuratewigner_amd import WignerBlochCUDA1D
+
+    def run():
+        g = WignerBlochCUDA1D(X_gridDIM=256, X_amplitude=8., P_gridDIM=256, P_amplitude=8., kT=0.5,
+                              dbeta=0.005, t_initial=0., t=0., V="0.5 * X * X + 0.1 * X * X * X * X",
+                              K="0.5 * P * P", precision=prec)
+        W = g.get_gibbs_state(record=[g.hamiltonian, "X * X", "P * P"]).get()
+        return g, W.astype(np.complex128)
+
+    a, Wa = run()
+    assert any(v is not None for v in a._gibbs_mw.values())     # the merged chunks ran
+    monkeypatch.setattr(WignerBlochCUDA1D, "_gibbs_merge", lambda self, dbeta, record: None)
+    b, Wb = run()
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    assert a.gibbs_stopped_at is None and b.gibbs_stopped_at is None
+    assert a.gibbs_record.shape == b.gibbs_record.shape == (400, 3)
+    assert np.max(np.abs(a.gibbs_record - b.gibbs_record) / np.abs(b.gibbs_record)) < tol
+    assert np.linalg.norm(Wa - Wb) < tol * np.linalg.norm(Wb)
+    assert abs(a.get_purity() - b.get_purity()) < tol

@@ -127,3 +127,58 @@ def test_pair_dot(prec):
     sv = np.einsum("bij,i->b", ab, v)
     ref = np.stack([su.real, su.imag, sv.real, sv.imag], axis=1)
     assert np.max(np.abs(out.get() - ref)) < _tol(prec) * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("shape", [(256, 128), (1024, 1024)])
+def test_column_pass_stores_partials_after_the_factor(prec, shape):
+    """obs_mode 1: a [forward, factor] column pass stores the partials of its output v (the
+    values right after the factor multiply), weighted sums and sum |v|^2 w2; a [forward,
+    factor, inverse] pass stores the same partials and outputs the inverse FFT of v."""
+    from accuratewigner_amd.device import empty, to_device
+    from accuratewigner_amd.engine import FWD, INV, Plan, finish_sums
+    rows, cols = shape
+    B, nw = 2, 3
+    rng = np.random.default_rng(6)
+    p = Plan(rows, cols, B, prec)
+    nblk = p.obs_blocks(0)
+    assert nblk > 0
+    z = _rand(rng, (B, rows, cols)).astype(p.state_dtype)
+    w = to_device(_rand(rng, (nw, rows, cols)))
+    w2h = rng.random((rows, cols)) + 0.5
+    w2 = to_device(w2h)
+    lf = to_device(np.exp(1j * rng.random(cols)).astype(p.state_dtype))   # one per column (line)
+    ef = to_device((np.exp(1j * rng.random(rows)) / rows).astype(p.state_dtype))
+    a, b = to_device(z), to_device(z)
+    pa, pb = empty((B, nblk, nw + 1), np.float64), empty((B, nblk, nw + 1), np.float64)
+    kw = dict(line_fac=lf, elem_fac=ef, obs_weights=w, obs_nw=nw, obs_mode=1, obs_norm2=1, obs_norm2_weights=w2)
+    p.run(a, 0, FWD, 0, obs_partials=pa, **kw)
+    p.run(b, 0, FWD, INV, obs_partials=pb, **kw)
+    ga, gb = empty((B, nw + 1), np.float64), empty((B, nw + 1), np.float64)
+    finish_sums(pa, nblk, nw + 1, B, ga)
+    finish_sums(pb, nblk, nw + 1, B, gb)
+    v = a.get().astype(np.complex128)
+    wh = w.get()
+    ref = np.concatenate([np.einsum("qij,bij->bq", np.conj(wh), v).real,
+                          np.einsum("bij,ij->b", np.abs(v) ** 2, w2h)[:, None]], axis=1)
+    # the partials add exactly the stored values in FP64: only the order of the sums differs
+    assert np.max(np.abs(ga.get() - ref)) < 1e-12 * np.max(np.abs(ref))
+    assert np.max(np.abs(gb.get() - ga.get())) < 1e-12 * np.max(np.abs(ref))
+    c = to_device(a.get())
+    p.run(c, 0, 0, INV)
+    assert np.max(np.abs(b.get() - c.get())) <= 1e-12 * np.max(np.abs(c.get())) + (1e-6 if prec == "mixed" else 0)
+
+
+def test_pass_partials_refused_where_blocks_are_not_whole_waves():
+    from accuratewigner_amd import _lib
+    from accuratewigner_amd.device import empty, to_device
+    from accuratewigner_amd.engine import FWD, Plan
+    p = Plan(64, 64, 1, "fp64")               # single-state 64-point columns: 2 x 16 threads
+    assert p.obs_blocks(0) == 0 and p.obs_blocks(1) > 0
+    z = to_device(np.ones((1, 64, 64), np.complex128))
+    w = to_device(np.ones((1, 64, 64), np.complex128))
+    part = empty((1, 64, 2), np.float64)
+    with pytest.raises(_lib.awigInvalidValue):
+        p.run(z, 0, FWD, 0, obs_weights=w, obs_nw=1, obs_partials=part, obs_mode=1, obs_norm2=1)
+    with pytest.raises(_lib.awigInvalidValue):     # partials of the output: row passes only
+        p.run(z, 0, FWD, 0, obs_weights=w, obs_nw=1, obs_partials=part)
