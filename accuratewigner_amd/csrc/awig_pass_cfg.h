@@ -21,7 +21,8 @@ constexpr int kLdsBudget = 160 * 1024;
 //    rows of 256 and of 1024+ points load their factor vectors with the data (PF, 4-7 %
 //    faster on MALL-resident batches; at 512 points it costs a wave per SIMD and 1.5 %; FP64
 //    rows go from 244 to 282 VGPRs and lose 40 %);
-//  * columns: 8 adjacent columns per block (64-byte row segments in FP32, 128 in FP64);
+//  * columns: 8 adjacent columns per block (64-byte row segments in FP32, 128 in FP64; FP64
+//    columns of 512 and 1024 points: 4, see C64);
 //  * twiddles in LDS (FP32 always, FP64 columns when they fit) or read through the cache;
 //  * the largest power-of-two LPB that fits the 160 KB LDS.
 // Measured at 1024^2 x 48 (FP32) / x 24 (FP64) states on MI355X.
@@ -43,9 +44,15 @@ template <typename R, int N, bool STRIDED> struct PassTune {
         return ((STRIDED ? (N + (N >> 4)) * lpb : lpb * (N + (N >> 5) + 1)) + (tw ? N : 0)) *
                int(sizeof(typename Cx<R>::T));
     }
+    // FP64 columns of 512 and 1024 points go 4 to a 256-thread block with the twiddles read
+    // through the cache (70 instead of 155 KB of LDS at 1024 points), so that two blocks
+    // share a CU and one block's loads and stores overlap the other's FFTs: 6.7e10 -> 7.35e10
+    // grid-point steps/s on 1024^2 x 48 (8 columns with the split exchange: 7.2e10; 2 columns:
+    // 7.1e10; 4 columns with both: no faster)
+    static constexpr bool C64 = !F32 && STRIDED && E == 16 && N >= 512 && N <= 1024;
     // keep LDS twiddles only if they do not cost lines per block
-    static constexpr int TWM = (TW_PREF && fit(WANT, 1) == fit(WANT, 0)) ? 1 : 0;
-    static constexpr int LPB = fit(WANT, TWM);
+    static constexpr int TWM = C64 ? 0 : ((TW_PREF && fit(WANT, 1) == fit(WANT, 0)) ? 1 : 0);
+    static constexpr int LPB = C64 ? 4 : fit(WANT, TWM);
     static constexpr int XS = (WIDE || (F32 && !STRIDED && E == 16)) ? 1 : 0;
     static constexpr int PF = (F32 && !STRIDED && (N == 256 || N >= 1024)) ? 1 : 0;
 };
