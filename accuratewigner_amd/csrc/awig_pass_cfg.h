@@ -48,7 +48,7 @@ template <typename R, int N, bool STRIDED> struct PassTune {
     // through the cache (70 instead of 155 KB of LDS at 1024 points), so that two blocks
     // share a CU and one block's loads and stores overlap the other's FFTs: 6.7e10 -> 7.35e10
     // grid-point steps/s on 1024^2 x 48 (8 columns with the split exchange: 7.2e10; 2 columns:
-    // 7.1e10; 4 columns with both: no faster)
+    // 7.1e10; 4 columns with both: no faster); at 512 points +2 % on 512^2 x 192, equal on x 48
     static constexpr bool C64 = !F32 && STRIDED && E == 16 && N >= 512 && N <= 1024;
     // keep LDS twiddles only if they do not cost lines per block
     static constexpr int TWM = C64 ? 0 : ((TW_PREF && fit(WANT, 1) == fit(WANT, 0)) ? 1 : 0);
