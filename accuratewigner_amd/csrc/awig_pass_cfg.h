@@ -53,7 +53,11 @@ template <typename R, int N, bool STRIDED> struct PassTune {
     // keep LDS twiddles only if they do not cost lines per block
     static constexpr int TWM = C64 ? 0 : ((TW_PREF && fit(WANT, 1) == fit(WANT, 0)) ? 1 : 0);
     static constexpr int LPB = C64 ? 4 : fit(WANT, TWM);
-    static constexpr int XS = (WIDE || (F32 && !STRIDED && E == 16)) ? 1 : 0;
+    // FP32 columns of 2048 points exchange in halves as well (86 -> 51 KB of LDS, 2 -> 3
+    // blocks per CU): 1.00e11 -> 1.17e11 grid-point steps/s on 2048^2 x 24.  At 4096 points
+    // that is 6 % slower, and FP64 columns of 2048 / 4096 points with half the columns per
+    // block and the twiddles through the cache lose 15 / 32 %.
+    static constexpr int XS = (WIDE || (F32 && STRIDED && N == 2048) || (F32 && !STRIDED && E == 16)) ? 1 : 0;
     static constexpr int PF = (F32 && !STRIDED && (N == 256 || N >= 1024)) ? 1 : 0;
 };
 
@@ -72,8 +76,9 @@ template <typename R, int N> constexpr long long small_lines_max() {
     return (N <= 256 || (sizeof(R) == 4 && N <= 512)) ? 2 * kSmallLines : kSmallLines;
 }
 
-// The step kinds load their factor vectors and mask with the data (PF = 2), so that their
-// latency overlaps the data loads instead of following the first FFT.
+// The step kinds of up to 8 elements per thread load their factor vectors and mask with the
+// data (PF = 2), so that their latency overlaps the data loads instead of following the first
+// FFT.
 // Columns of 1024 points go 4 to a block (32 / 64-byte row segments): the von Neumann step
 // on one 1024^2 state (three passes) takes 64.3 instead of 68.1 us in FP64 and 45.8 instead
 // of 49.6 us in mixed precision.  At 256 and 512 points, and with 1 or 8 columns at 1024,
@@ -82,7 +87,9 @@ template <typename R, int N, bool STRIDED> struct SmallTune {
     static constexpr int E = (N / 128 < 4) ? 4 : ((N / 128 > 16) ? 16 : N / 128);
     static constexpr int T = N / E;
     static constexpr int LPB = STRIDED ? (N == 1024 ? 4 : 2) : (T >= 64 ? 1 : 64 / T);
-    static constexpr int PF = 2;
+    // (E <= 8: with 16 elements FP64 passes of 2048 points would go to 240-256 VGPRs and one
+    // wave per SIMD)
+    static constexpr int PF = E <= 8 ? 2 : 0;
 };
 
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0>
