@@ -182,3 +182,12 @@ def test_pass_partials_refused_where_blocks_are_not_whole_waves():
         p.run(z, 0, FWD, 0, obs_weights=w, obs_nw=1, obs_partials=part, obs_mode=1, obs_norm2=1)
     with pytest.raises(_lib.awigInvalidValue):     # partials of the output: row passes only
         p.run(z, 0, FWD, 0, obs_weights=w, obs_nw=1, obs_partials=part)
+    # argument checks, before any launch
+    from accuratewigner_amd.engine import INV
+    w2 = to_device(np.ones((64, 64)))
+    for kw in (dict(obs_nw=1, obs_mode=2),                   # no such mode
+               dict(obs_nw=8, obs_norm2=1),                  # more than 8 components
+               dict(obs_nw=1, obs_norm2=2),
+               dict(obs_nw=1, obs_norm2_weights=w2)):        # weights without the component
+        with pytest.raises(_lib.awigInvalidValue):
+            p.run(z, 1, FWD, INV, obs_weights=w, obs_partials=part, **kw)

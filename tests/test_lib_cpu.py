@@ -91,6 +91,7 @@ def test_entry_points_reject_null_and_bad_arguments_before_touching_the_gpu():
         (_lib.awigInvalidPlan, lambda: lib.aw_step_sequence(None, vp(1), ctypes.byref(steps), None)),
         (_lib.awigInvalidPlan, lambda: lib.aw_pass_run(None, vp(1), ctypes.byref(ps), None)),
         (_lib.awigInvalidPlan, lambda: lib.aw_row_obs_blocks(None, ctypes.byref(ctypes.c_int()))),
+        (_lib.awigInvalidPlan, lambda: lib.aw_pass_obs_blocks(None, 0, ctypes.byref(ctypes.c_int()))),
         (_lib.awigInvalidPlan, lambda: lib.aw_step_group(None, 4, 0, ctypes.byref(ctypes.c_int()))),
         (_lib.awigInvalidPlan, lambda: lib.aw_pair_dot(None, vp(1), vp(1), vp(1), vp(1), vp(1), None)),
         (_lib.awigInvalidPlan, lambda: lib.aw_cyclic_diag_sums(None, vp(1), vp(1), None)),
