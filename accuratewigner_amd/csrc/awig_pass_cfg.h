@@ -92,10 +92,13 @@ template <typename R, int N, bool STRIDED> struct SmallTune {
     static constexpr int PF = E <= 8 ? 2 : 0;
 };
 
+// Column exchanges in real and imaginary halves pad one row per 32 positions (PADC 5)
+// instead of 16: +2 % on 1024^2 x 96 mixed (8: +0.7 %, 64 and 128: slower); the row pad
+// stays at 32 positions (16 is 3 % slower, 64 no faster).
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0>
 using PassK = KCfg<R, N, PassTune<R, N, STRIDED>::E, PassTune<R, N, STRIDED>::LPB, STRIDED,
-                   PassTune<R, N, STRIDED>::TWM, DA, DB, GEN, 1, PassTune<R, N, STRIDED>::PF, 4, 5, OBS,
-                   PassTune<R, N, STRIDED>::XS>;
+                   PassTune<R, N, STRIDED>::TWM, DA, DB, GEN, 1, PassTune<R, N, STRIDED>::PF,
+                   (PassTune<R, N, STRIDED>::XS && STRIDED) ? 5 : 4, 5, OBS, PassTune<R, N, STRIDED>::XS>;
 
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0>
 using PassKS = KCfg<R, N, SmallTune<R, N, STRIDED>::E, SmallTune<R, N, STRIDED>::LPB, STRIDED, 1, DA,

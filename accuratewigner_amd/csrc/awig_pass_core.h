@@ -151,8 +151,9 @@ __device__ __forceinline__ void dft_reg(T2 *x) {
 // LDS layouts of the line buffers.
 //  rows (lanes of a wave walk one line): one region per line, one pad slot per 32 positions;
 //  columns (lanes interleave LPB adjacent lines): position-major [p][l] with one pad row per
-//  16 positions, so both the natural-order reads (u + T k) and the Stockham scatter writes
-//  (16 u + r) of a wave hit distinct banks.
+//  2^PADC positions (16 for complex exchanges, 32 for exchanges in real and imaginary
+//  halves), so both the natural-order reads (u + T k) and the Stockham scatter writes
+//  (16 u + r) of a wave spread over the banks.
 template <bool STRIDED, int N, int LPB, int PADC = 4, int PADR = 5> struct LdsLayout {
     static constexpr int REGION = N + (N >> PADR) + 1;
     static constexpr int SIZE = STRIDED ? (N + (N >> PADC)) * LPB : LPB * REGION;
