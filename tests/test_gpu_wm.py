@@ -175,3 +175,21 @@ def test_wm_batch_equals_separate_runs(precision):
         g1.set_wignerfunction(W0[k])
         W1 = g1.propagate(40).get()
         assert np.array_equal(Wb[k], W1), k
+
+
+def test_batched_2048_grid_mixed_matches_fp64():
+    """2048^2 batches run the throughput configurations of the step passes (FP32 columns of
+    2048 points exchange in real and imaginary halves): a short mixed-precision propagation
+    stays within FP32 rounding of the FP64 one."""
+    from accuratewigner_amd import WignerMoyalCUDA1D
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s = WignerMoyalCUDA1D(X_gridDIM=2048, X_amplitude=10., P_gridDIM=2048, P_amplitude=10., dt=0.01, t=0.,
+                              K="0.5 * P * P", V="0.5 * X * X + 0.05 * X * X * X * X", precision=prec, batch=2)
+        X, P = s.X, s.P
+        W0 = np.stack([np.exp(-(X - x0) ** 2 - (P - 0.3) ** 2) / np.pi for x0 in (1.0, -1.5)])
+        s.set_wignerfunction(W0)
+        out[prec] = s.propagate(20).get().astype(np.complex128)
+        del s
+    assert out["fp64"].shape == (2, 2048, 2048)
+    assert rel(out["mixed"], out["fp64"]) < 2e-5
