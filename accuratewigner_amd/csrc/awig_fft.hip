@@ -366,7 +366,7 @@ __global__ __launch_bounds__(256) void k_cyclic_diag(const void *data, int n, do
     const T2 *z = reinterpret_cast<const T2 *>(data) + (long long)b * n * n;
     double re = 0.0, im = 0.0;
     if (d < n) {
-#pragma unroll 4
+#pragma unroll   // all row loads in flight at once (one 1024^2 step: -0.6 us)
         for (int i = r0; i < r0 + kDiagRowsPerBlock; ++i) {
             const T2 q = z[(long long)i * n + ((i + d) & (n - 1))];
             re += q.x;
