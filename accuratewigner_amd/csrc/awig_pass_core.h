@@ -310,10 +310,12 @@ __device__ __forceinline__ void obs_block_partials(const PassArgs &a, const T2 *
     const int lane = tid & 63, wid = tid >> 6;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        double s = acc[q];
+        if (q < nc) {   // nc is uniform: only the components in use are reduced
+            double s = acc[q];
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-        if (lane == 0) ored[wid][q] = s;
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+            if (lane == 0) ored[wid][q] = s;
+        }
     }
     __syncthreads();
     if (tid < nc) {
