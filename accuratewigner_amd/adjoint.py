@@ -106,12 +106,12 @@ class WignerMoyalDipoleControl:
                                           sign * self.dt, merged=merged, k_stride=0)
         return f, (Vj, Vi)
 
-    def _segment_factors(self, E, n0, n1):
-        """Fill the persistent V tables and factor vectors of steps n0..n1-1 (both signs)."""
-        e = E[self.piece[n0:n1]]
+    def _segment_factors(self, Vj_all, Vi_all, n0, n1):
+        """Fill the persistent V tables and factor vectors of steps n0..n1-1 (both signs),
+        copying the tables on the device from those of all steps."""
         s, m = self.sys, n1 - n0
-        self._segV[0][:m] = self._V0j[None, :] + e[:, None] * s._kXX[None, :]
-        self._segV[1][:m] = self._V0i[None, :] + e[:, None] * s._kXXp[None, :]
+        self._segV[0].tensor[:m].copy_(Vj_all.tensor[n0:n1])
+        self._segV[1].tensor[:m].copy_(Vi_all.tensor[n0:n1])
         for sg in (1, -1):
             s._factors_from_tables(m, self._segV[0].ptr, self._segV[1].ptr, self._Kj.ptr, self._Ki.ptr,
                                    sg * self.dt, merged=False, k_stride=0, dst=self._segF[sg])
@@ -153,16 +153,22 @@ class WignerMoyalDipoleControl:
         s.t = self.t0
         s.set_wignerfunction(self.W0)
         self._ckpt.tensor[0].copy_(s.rho.tensor)
+        # the V tables of all steps in one upload: a host-to-device copy waits for the work
+        # already queued, so one per chunk would make the host and the GPU take turns
+        Vj_all, Vi_all = self._tables(E, 0, self.steps)
+        Nx, Np = s.X_gridDIM, s.P_gridDIM
         for c in range(self._nck):
             n0, n1 = c * K, min(self.steps, (c + 1) * K)
-            (colL, colE, rowL, rowE), keep = self._factors(E, n0, n1, 1.0, merged=True)
+            colL, colE, rowL, rowE = s._factors_from_tables(n1 - n0, Vj_all.ptr + n0 * Nx * 8, Vi_all.ptr + n0 * Np * 8,
+                                                            self._Kj.ptr, self._Ki.ptr, self.dt, merged=True,
+                                                            k_stride=0)
             p.steps(s.rho, n1 - n0, colL, colE, rowL, rowE)
             self._ckpt.tensor[c + 1].copy_(s.rho.tensor)
         s.t = self.t0 + self.steps * self.dt
         s.rho2wigner_()
         norm = float(p.stats(s.wignerfunction).get()[0, 0]) * s.dXdP
         J = s.get_average("X") / norm
-        self._state = (E.copy(), J, norm)
+        self._state = (E.copy(), J, norm, (Vj_all, Vi_all))
         return J
 
     # -- backward --------------------------------------------------------------------------
@@ -175,7 +181,7 @@ class WignerMoyalDipoleControl:
                 self.forward(E)
         if self._state is None:
             raise RuntimeError("call forward(E) first")
-        E, J, norm = self._state
+        E, J, norm, (Vj_all, Vi_all) = self._state
         s, p, K = self.sys, self.sys.plan, self.checkpoint
         g = s._grid
         # lambda_N = L^H f with f_ij = (X_j - J) dXdP / norm (L = rho2wigner)
@@ -186,7 +192,7 @@ class WignerMoyalDipoleControl:
             n0, n1 = c * K, min(self.steps, (c + 1) * K)
             m, last = n1 - n0, c == self._nck - 1
             self._seg.tensor[0].copy_(self._ckpt.tensor[c])
-            self._segment_factors(E, n0, n1)
+            self._segment_factors(Vj_all, Vi_all, n0, n1)
             graphs.run_chunk(self._graphs, (m, last), lambda m=m, last=last: self._segment_body(m, last), True)
             rows = m + 1 if last else m
             self._dots.tensor[n0:n0 + rows].copy_(self._segdots.tensor[:rows])
