@@ -243,7 +243,10 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
             Kt = empty((n, N), np.float64)
         else:
             ts_dev, Vt, Kt = bufs["ts"], bufs["Vt"], bufs["Kt"]
-            ts_dev[:n] = ts
+            if isinstance(ts, np.ndarray):
+                ts_dev[:n] = ts
+            else:
+                ts_dev.tensor[:n].copy_(ts)   # a device slice: ordered on the stream, no host wait
         self._eval_1d(0, self._dev_kX, N, ts_dev, n, 0.5 * dt, Vt)
         self._eval_1d(1, self._dev_kP, N, ts_dev, n, 0.5 * dt, Kt)
         bx = self._bx.ptr if self._bx is not None else None
@@ -365,10 +368,10 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
             self._dscale.tensor[:, 0] = 1.0
             self._dscale.tensor[:, 1] = 0.0
             self._scale_pending = True
+        ts_all = to_device(ts)   # one upload; each chunk copies its slice on the device
         for c0 in range(0, steps, CH):
-            tsc = ts[c0:c0 + CH]
-            n = len(tsc)
-            self._chunk_factors(tsc, merged=False, bufs=bufs)
+            n = min(CH, steps - c0)
+            self._chunk_factors(ts_all.tensor[c0:c0 + n], merged=False, bufs=bufs)
             # the observable weights of all the chunk's times at once
             self._weights_t(exprs_x, bufs["ts"], n, bufs["xw"])
             self._weights_t(exprs_p, bufs["ts"], n, bufs["pwc"], complex_out=True)

@@ -502,7 +502,10 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             dst = None
         else:
             ts_dev = bufs["ts"]
-            ts_dev[:n] = ts
+            if isinstance(ts, np.ndarray):
+                ts_dev[:n] = ts
+            else:
+                ts_dev.tensor[:n].copy_(ts)   # a device slice: ordered on the stream, no host wait
             Vj, Vi, Kj, Ki = bufs["Vj"], bufs["Vi"], bufs["Kj"], bufs["Ki"]
             dst = bufs["fac"]
         self._eval_1d(0, self._dev_kXX, Nx, ts_dev, n, 0.5 * dt, Vj)
@@ -603,6 +606,9 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
                 self.normalize_rho_wigner()
             return self.wignerfunction
         ts = self._times(steps)
+        # the step times of the whole call in one upload; each chunk copies its slice on the
+        # device (a host upload per chunk would wait for the chunk before it)
+        ts_all = to_device(ts)
         if not self.isEhrenfest:
             self._flush_scale()
             CH = CHUNK_STEPS
@@ -611,9 +617,8 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             # latency-bound sizes (the small-pass kernels) replay a captured chunk
             small = self.batch * max(self.X_gridDIM, self.P_gridDIM) <= 2048
             for c0 in range(0, steps, CH):
-                tsc = ts[c0:c0 + CH]
-                n = len(tsc)
-                self._build_chunk_factors(tsc, merged=True, bufs=self._prop)
+                n = min(CH, steps - c0)
+                self._build_chunk_factors(ts_all.tensor[c0:c0 + n], merged=True, bufs=self._prop)
                 ev = self._pass_events
                 if ev is not None:
                     import torch
@@ -660,9 +665,8 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             self._scale_pending = True
         chunk = self._ehrenfest_merged_chunk if merged else self._ehrenfest_chunk
         for c0 in range(0, steps, CH):
-            tsc = ts[c0:c0 + CH]
-            n = len(tsc)
-            self._build_chunk_factors(tsc, merged=merged, bufs=bufs)
+            n = min(CH, steps - c0)
+            self._build_chunk_factors(ts_all.tensor[c0:c0 + n], merged=merged, bufs=bufs)
             graphs.run_chunk(bufs["graphs"], (merged,) + tuple(exprs), lambda n=n: chunk(exprs, n), n == CH)
             ring.tensor[c0:c0 + n].copy_(bufs["stage"].tensor[:n])
         self.t = ts[-1]
