@@ -278,37 +278,29 @@ class WignerBlochCUDA1D(WignerMoyalCUDA1D):
             gkey = (float(dbeta), tuple(record), float(self.t), mw is not None)
         rec = []
         k = 0
+        if fast:
+            self._gibbs_fast_chunks(dbeta, num_beta_steps, fb, gkey, g, mw, m, nc, rec)
+            if record:
+                self.gibbs_record = np.concatenate(rec, axis=0) if rec else np.zeros((0, m))
+            return self.wignerfunction
+        # the loop that forms W every step (obs_by_adjoint off, or more than 6 recorded sums)
         while k < num_beta_steps:
             n = min(CHECK_CHUNK, num_beta_steps - k)
             saved = self.rho.copy()
             saved_scale = self._scale_pending
             saved_ds = self._dscale.copy()
-            stats = fb["stats"] if fast else empty((n, self.batch, nc), np.float64)
-            ring = empty((n, self.batch, m), np.float64) if (record and not fast) else None
-
-            def step(s):
-                if fast:
-                    # [Re sum W, recorded sums, sum |rho|^2] and the scale, one launch
-                    self._normalized_step_sums(dbeta, g, m + 1, stats.ptr + s * self.batch * nc * 8)
-                    return
-                self._normalized_step(dbeta, stats[s])
-                if record:
-                    self._obs_launch(record, self.wignerfunction, float(self.t), ring.ptr + s * self.batch * m * 8)
+            stats = empty((n, self.batch, nc), np.float64)
+            ring = empty((n, self.batch, m), np.float64) if record else None
 
             def steps(n_):
-                if fast and mw is not None:
-                    self._gibbs_merged_chunk(dbeta, n_, mw, fb)
-                else:
-                    for s in range(n_):
-                        step(s)
+                for s in range(n_):
+                    self._normalized_step(dbeta, stats[s])
+                    if record:
+                        self._obs_launch(record, self.wignerfunction, float(self.t), ring.ptr + s * self.batch * m * 8)
 
-            if fast:
-                graphs.run_chunk(fb["graphs"], gkey, lambda n=n: steps(n), n == CHECK_CHUNK)
-            else:
-                steps(n)
+            steps(n)
             st = stats.get()[:n]
-            pur = (self._purities(st, n, nc, m + 1, float(self.P_gridDIM)) if fast
-                   else self._purities(st, n))
+            pur = self._purities(st, n)
             bad = np.nonzero(~(pur < self.purity_limit))[0]
             if len(bad):
                 stop = int(bad[0])
@@ -317,20 +309,73 @@ class WignerBlochCUDA1D(WignerMoyalCUDA1D):
                 self._scale_pending = saved_scale
                 self._dscale[:] = saved_ds
                 steps(stop + 1)
-                self._finish_gibbs(fast)
+                self._finish_gibbs(False)
                 self.gibbs_stopped_at = k + stop
                 print("Warning: Gibbs state calculations interupted because purity = %.10f > 1."
                       "Current kT = %.6f" % (float(np.max(pur[stop])), 1 / (max(k + stop, 1) * dbeta)))
                 if record:
-                    rec.append(self._gibbs_record_values(fast, st, ring, m, stop + 1))
+                    rec.append(self._gibbs_record_values(False, st, ring, m, stop + 1))
                 break
             if record:
-                rec.append(self._gibbs_record_values(fast, st, ring, m, n))
+                rec.append(self._gibbs_record_values(False, st, ring, m, n))
             k += n
-        self._finish_gibbs(fast)
+        self._finish_gibbs(False)
         if record:
             self.gibbs_record = np.concatenate(rec, axis=0) if rec else np.zeros((0, m))
         return self.wignerfunction
+
+    def _gibbs_fast_chunks(self, dbeta, num_beta_steps, fb, gkey, g, mw, m, nc, rec):
+        """The Gibbs loop on the sums from rho, one chunk ahead: chunk c + 1 is queued
+        before the host checks the purities of chunk c, whose sums are copied to pinned host
+        memory behind it, so the GPU does not wait for the check.  A violation in chunk c
+        restores the state saved before chunk c (the queued chunk is discarded) and replays
+        chunk c up to the violating step, as the one-chunk-at-a-time loop does."""
+        import torch
+        B, Np = self.batch, float(self.P_gridDIM)
+
+        def steps(n_):
+            if mw is not None:
+                self._gibbs_merged_chunk(dbeta, n_, mw, fb)
+            else:
+                for s in range(n_):
+                    # [Re sum W, recorded sums, sum |rho|^2] and the scale, one launch
+                    self._normalized_step_sums(dbeta, g, m + 1, fb["stats"].ptr + s * B * nc * 8)
+
+        def launch(k0):
+            n = min(CHECK_CHUNK, num_beta_steps - k0)
+            saved, saved_ds = self.rho.copy(), self._dscale.copy()
+            graphs.run_chunk(fb["graphs"], gkey, lambda n=n: steps(n), n == CHECK_CHUNK)
+            host = torch.empty((n, B, nc), dtype=torch.float64, pin_memory=True)
+            host.copy_(fb["stats"].tensor[:n], non_blocking=True)
+            done = torch.cuda.Event()
+            done.record()
+            return k0, n, saved, saved_ds, host, done
+
+        cur = launch(0)
+        while cur is not None:
+            k, n, saved, saved_ds, host, done = cur
+            cur = launch(k + n) if k + n < num_beta_steps else None
+            done.synchronize()
+            st = host.numpy()
+            pur = self._purities(st, n, nc, m + 1, Np)
+            bad = np.nonzero(~(pur < self.purity_limit))[0]
+            if len(bad):
+                stop = int(bad[0])
+                # replay up to and including the violating step (deterministic)
+                self.rho[:] = saved
+                self._scale_pending = True
+                self._dscale[:] = saved_ds
+                steps(stop + 1)
+                self._finish_gibbs(True)
+                self.gibbs_stopped_at = k + stop
+                print("Warning: Gibbs state calculations interupted because purity = %.10f > 1."
+                      "Current kT = %.6f" % (float(np.max(pur[stop])), 1 / (max(k + stop, 1) * dbeta)))
+                if m:
+                    rec.append(self._gibbs_record_values(True, st, None, m, stop + 1))
+                return
+            if m:
+                rec.append(self._gibbs_record_values(True, st, None, m, n))
+        self._finish_gibbs(True)
 
     def _finish_gibbs(self, fast):
         """Normalise rho and W by the last step's scale (W is formed here when the steps
