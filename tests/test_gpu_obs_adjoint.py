@@ -129,3 +129,26 @@ def test_gibbs_merged_chunks_match_the_per_step_sums(prec, monkeypatch):
     assert np.max(np.abs(a.gibbs_record - b.gibbs_record) / np.abs(b.gibbs_record)) < tol
     assert np.linalg.norm(Wa - Wb) < tol * np.linalg.norm(Wb)
     assert abs(a.get_purity() - b.get_purity()) < tol
+
+
+def test_gibbs_merged_chunks_batch_of_two():
+    """The merged Gibbs chunks on a batch of two states (partials ring [step][item][block],
+    one finish over step x item): both items equal the loop that forms W every step."""
+    from accuratewigner_amd import WignerBlochCUDA1D
+
+    def run(by_adjoint):
+        g = WignerBlochCUDA1D(X_gridDIM=128, X_amplitude=8., P_gridDIM=128, P_amplitude=8., kT=0.5,
+                              dbeta=0.005, t_initial=0., t=0., V="0.5 * X * X + 0.1 * X * X * X * X",
+                              K="0.5 * P * P", batch=2)
+        g.obs_by_adjoint = by_adjoint
+        W = g.get_gibbs_state(record=[g.hamiltonian, "X * X"]).get()
+        return g, W
+
+    a, Wa = run(True)
+    assert any(v is not None for v in a._gibbs_mw.values())
+    b, Wb = run(False)
+    assert a.gibbs_record.shape == b.gibbs_record.shape == (400, 2, 2)
+    assert np.max(np.abs(a.gibbs_record - b.gibbs_record) / np.abs(b.gibbs_record)) < 1e-12
+    assert Wa.shape == (2, 128, 128)
+    assert np.linalg.norm(Wa - Wb) < 1e-12 * np.linalg.norm(Wb)
+    assert np.linalg.norm(Wa[0] - Wa[1]) < 1e-14 * np.linalg.norm(Wa[0])
