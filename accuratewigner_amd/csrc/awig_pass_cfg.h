@@ -10,7 +10,8 @@ constexpr int kLdsBudget = 160 * 1024;
 // Per (precision, line length, line kind):
 //  * E = 16 elements per thread (8 below 256), so a 1024-point line is 64 lanes, radix 16-16-4;
 //    FP32 columns of 512 and 1024 points use E = 32 (radix 32-16 / 32-32: one LDS exchange,
-//    8 columns per 128 / 256-thread block; 14% faster at 1024^2 x 96 than E = 16), and
+//    8 columns per 128 / 256-thread block; 14% faster at 1024^2 x 96 than E = 16, and still
+//    8 % faster than E = 16 with the split exchange and 32-position pads), and
 //    exchange the real and the imaginary parts in turn (43 instead of 76 KB of LDS; 6% faster
 //    on a MALL-resident 24-state group of 1024^2, 2% slower on 96 states from HBM);
 //  * FP32 rows of E = 16 exchange the same way: a 1024-point row block then takes 25 instead
