@@ -889,22 +889,40 @@ class SchrodingerWignerOracle:
             self.stationary_states.append(self.wavefunction.copy())
         return self
 
-    def get_wigner(self):
-        """:593-625, with the missing array and plans created (the reference never allocates
-        self.wignerfunction nor plan_Z2Z_ax0/1, :244, :255-256)."""
-        F, I = self.dft.F, self.dft.I
+    def wigner_shear_factors(self):
+        """phase_shearX / phase_shearY of the Schrodinger class (:695-726): kernel P in FFT
+        order, X_prime / P_prime on the row index."""
         N = self.X_gridDIM
-        psi = self.wavefunction[0]
-        w = psi[None, :] * np.conj(psi)[:, None]                  # psi2rho :680-687
         a = np.tan(np.pi / 8.)
         b = -np.sin(np.pi / 4.)
         SX = np.exp(1j * (-a * self.kP[None, :] * self.kX[:, None])) / float(N)   # :695-710
         SY = np.exp(1j * (-b * self.kP[:, None] * self.kX[None, :])) / float(N)   # :712-726
-        w = I(SX * F(w, 1), 1)
-        w = I(SY * F(w, 0), 0)
-        w = I(SX * F(w, 1), 1)
+        return SX, SY
+
+    def blackman_factors(self):
+        """blackmanX (along j) and blackmanY (along i) of the Schrodinger class (:728-750); both
+        use X_gridDIM (the grid is square)."""
+        c = np.arange(self.X_gridDIM) * 2.0 * np.pi / (self.X_gridDIM - 1.)
+        w = 0.42 - 0.5 * np.cos(c) + 0.08 * np.cos(2.0 * c)
+        return w[None, :], w[:, None]
+
+    def get_wigner(self, blackman=False):
+        """:593-625 (``blackman=True``: get_wigner_blackman, :627-663), with the missing array
+        and plans created (the reference never allocates self.wignerfunction nor
+        plan_Z2Z_ax0/1, :244, :255-256).  W[i, j] is the Wigner function at
+        (P_wigner[i], X_wigner[j]), normalised so that sum W * wigner_dXdP = 1 with
+        wigner_dXdP = dX dP, twice the cell area of that grid."""
+        F, I = self.dft.F, self.dft.I
+        N = self.X_gridDIM
+        psi = self.wavefunction[0]
+        w = psi[None, :] * np.conj(psi)[:, None]                  # psi2rho :680-687
+        SX, SY = self.wigner_shear_factors()
+        bx, by = self.blackman_factors() if blackman else (1.0, 1.0)
+        w = I(SX * F(bx * w, 1), 1)
+        w = I(SY * F(by * w, 0), 0)
+        w = I(SX * F(bx * w, 1), 1)
         srow = (1 - 2 * (np.arange(N) % 2)).astype(np.float64)[:, None]           # :763-771
-        w = srow * I(srow * w, 0)
+        w = srow * I(srow * (by * w), 0)
         w /= w.sum().real * self.wigner_dXdP
         self.wignerfunction = w
         return w
