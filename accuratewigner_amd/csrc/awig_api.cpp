@@ -463,15 +463,10 @@ int aw_sum_partials(const double *partials, int nblk, int ncomp, int batch, doub
 int aw_reduce_vec(int prec, const void *data, long long n, int batch, double *out, aw_stream stream) {
     if (!data || !out || n < 1 || batch < 1) return set_error(AW_ERR_INVALID_VALUE, "bad vector reduction");
     if (prec != AW_PREC_F64 && prec != AW_PREC_F32) return set_error(AW_ERR_INVALID_TYPE, "bad precision");
-    long long nb = (n + 256 * 8 - 1) / (256 * 8);
-    if (nb > 1024) nb = 1024;
-    double *partials = nullptr;
-    // small temporary; vector reductions are off the step hot path
-    AW_HIP(hipMallocAsync((void **)&partials, sizeof(double) * 8 * nb * batch, (hipStream_t)stream));
-    int rc = launch_stats(prec, data, n, 1, (int)(n > 0x7fffffff ? 0x7fffffff : n), batch, partials,
-                          (int)nb, out, (hipStream_t)stream);
-    (void)hipFreeAsync(partials, (hipStream_t)stream);
-    return rc;
+    // one block per item in a fixed order, no scratch (nothing is allocated; a plan's
+    // aw_reduce_stats splits large items over many blocks)
+    return launch_stats(prec, data, n, 1, (int)(n > 0x7fffffff ? 0x7fffffff : n), batch, nullptr, 1, out,
+                        (hipStream_t)stream);
 }
 
 int aw_scale_from_stats(const double *stats, int stats_stride, int comp, int mode, double factor,

@@ -390,6 +390,18 @@ __global__ void k_sum_partials(const double *partials, int nblk, int ncomp, doub
 
 int launch_stats(int prec, const void *data, long long n_item, int rows, int cols, int batch,
                  double *partials, int nblk, double *out, hipStream_t st) {
+    if (!partials) {
+        // one block per item writes its sums straight to out: no scratch, one launch
+        if (prec == AW_PREC_F64)
+            hipLaunchKernelGGL(k_stats<double>, dim3(1, batch), dim3(RED_THREADS), 0, st, data, n_item, rows,
+                               cols, out, 1);
+        else
+            hipLaunchKernelGGL(k_stats<float>, dim3(1, batch), dim3(RED_THREADS), 0, st, data, n_item, rows,
+                               cols, out, 1);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_stats: %s", hipGetErrorString(e));
+        return AW_OK;
+    }
     if (prec == AW_PREC_F64)
         hipLaunchKernelGGL(k_stats<double>, dim3(nblk, batch), dim3(RED_THREADS), 0, st, data, n_item,
                            rows, cols, partials, nblk);

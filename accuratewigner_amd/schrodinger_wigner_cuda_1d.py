@@ -26,7 +26,7 @@ import numpy as np
 from . import _lib, rtc
 from ._params import KeywordParams
 from .device import DeviceArray, current_stream, empty, require_gpu, to_device
-from .engine import FWD, INV, Plan, mem_info, reduce_vec
+from .engine import FWD, INV, Plan, mem_info
 
 SQRT2 = np.sqrt(2.0)
 CHUNK_STEPS = 256
@@ -106,12 +106,14 @@ class SchrodingerWignerCUDA1D(KeywordParams):
                                                                                   self.precision)
         self.state_dtype = self.plan.state_dtype
         self._wplan = None
+        self._blackman = None
         self._mod = rtc.Module(self._physics_source(), "schrodinger_physics.hip")
         self._dev_kX = to_device(self._kX)
         self._dev_kP = to_device(self._kP_fac)
         self.wavefunction = DeviceArray((1, N), self.state_dtype)
         self._tmp = DeviceArray((1, N), self.state_dtype)
         self._dscale = empty((1, 2), np.float64)
+        self._norm_st = empty((1, 8), np.float64)
         self._scale_pending = False
         self._obs_cache = {}
         self._masks()
@@ -182,10 +184,11 @@ extern "C" __global__ void aw_outer(const %s *psi, %s *rho, int n)
     # norms and observables
     # ------------------------------------------------------------------------------------
     def _norm_scale(self, factor=1.0, target=None):
-        """dscale = 1 / (||target|| * factor) on the device (the cu_linalg.norm divisions)."""
+        """dscale = 1 / (||target|| * factor) on the device (the cu_linalg.norm divisions).
+        The sums use the plan's own block partials and a persistent output row: nothing is
+        allocated per call."""
         target = self.wavefunction if target is None else target
-        st = empty((1, 8), np.float64)
-        reduce_vec(self.plan.prec, target, self.X_gridDIM, 1, st)
+        st = self.plan.stats(target, out=self._norm_st)
         mod = self._obs_module([])
         mod.launch("aw_inv_norm", (1,), (1,), [st.ptr, ctypes.c_double(factor), self._dscale.ptr],
                    current_stream())
@@ -553,7 +556,20 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
     # ------------------------------------------------------------------------------------
     def get_wigner(self):
         """:593-625 with its missing N x N array and plans created: psi2rho; [F1, SX, I1];
-        [F0, SY, I0]; [F1, SX, I1]; (-1)^i, I0, (-1)^i; W /= Re sum W dX dP."""
+        [F0, SY, I0]; [F1, SX, I1]; (-1)^i, I0, (-1)^i; W /= Re sum W dX dP.  W[i, j] is the
+        Wigner function at (P_wigner[i], X_wigner[j]) divided by 2: the reference divides by
+        sum W * dX dP, twice the cell area of that grid."""
+        return self._wigner(blackman=False)
+
+    def get_wigner_blackman(self):
+        """:627-663: get_wigner with a Blackman window (0.42 - 0.5 cos c + 0.08 cos 2c,
+        c = 2 pi k / (N - 1), :728-750) before each forward FFT: along j before the X shears,
+        along i before the Y shear and the final inverse FFT.  A window that precedes a pass's
+        first FFT is its own multiply pass; the last one is the factor of [(-1)^i, by, I0,
+        (-1)^i]."""
+        return self._wigner(blackman=True)
+
+    def _wigner(self, blackman):
         self._flush_scale()
         N = self.X_gridDIM
         if self._two is not None:
@@ -572,10 +588,23 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
                    scale=1.0 / N)
         shy = dict(chirp=b, lstep=kdx, lcenter=0.5 * N, estep=kdp, ecenter=0.5 * N, efft=1,
                    scale=1.0 / N)
-        wp.run(W, 1, FWD, INV, **shx)
-        wp.run(W, 0, FWD, INV, **shy)
-        wp.run(W, 1, FWD, INV, **shx)
-        wp.run(W, 0, INV, 0, sign_in=2, sign_out=2)
+        if blackman:
+            if self._blackman is None:
+                c = np.arange(N) * 2.0 * np.pi / (N - 1.)
+                self._blackman = to_device((0.42 - 0.5 * np.cos(c) + 0.08 * np.cos(2.0 * c)).astype(self.state_dtype))
+            win = self._blackman
+            wp.run(W, 1, 0, 0, elem_fac=win)
+            wp.run(W, 1, FWD, INV, **shx)
+            wp.run(W, 0, 0, 0, elem_fac=win)
+            wp.run(W, 0, FWD, INV, **shy)
+            wp.run(W, 1, 0, 0, elem_fac=win)
+            wp.run(W, 1, FWD, INV, **shx)
+            wp.run(W, 0, 0, INV, elem_fac=win, sign_in=2, sign_out=2)
+        else:
+            wp.run(W, 1, FWD, INV, **shx)
+            wp.run(W, 0, FWD, INV, **shy)
+            wp.run(W, 1, FWD, INV, **shx)
+            wp.run(W, 0, INV, 0, sign_in=2, sign_out=2)
         st = wp.stats(W)
         wp.scale_from_stats(st.ptr, 8, 0, 0, self.wigner_dXdP, self._wscale)
         wp.scale(W, self._wscale)

@@ -133,6 +133,7 @@ class WignerMoyalCUDA1D(KeywordParams):
             self.hamiltonian_average = []
         self._observable_kernels = {}
         self._obs_partials = {}
+        self._g1 = None           # L^H 1: the norm weights of normalize_rho_wigner
         self._ehr = None          # persistent chunk buffers and captured graphs of the Ehrenfest loop
         self._prop = None         # the same for the merged-step loop
         self._prepare_masks()
@@ -457,13 +458,32 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             self.plan.scale(self.rho, self._dscale)
             self._scale_pending = False
 
+    def _norm_weights(self):
+        """L^H 1 (FP64 [1, Np, Nx]) with L = rho2wigner_: Re <L^H 1, rho> = Re sum W."""
+        if self._g1 is None:
+            keep = getattr(self, "_obs_plan", None)
+            self._g1 = self._obs_weights(["1."])
+            self._g1_plan, self._obs_plan = self._obs_plan, keep   # the plan outlives its queued launches
+            self._norm_out = empty((self.batch, 1), np.float64)
+        return self._g1
+
     def normalize_rho_wigner(self):
         """wigner_moyal_cuda_1d.py:608-622: W = rho2wigner; n = Re sum W dXdP; W /= n;
-        rho /= n (device-side, no host sync)."""
-        self.rho2wigner()
-        st = self.plan.stats(self.wignerfunction)
-        self.plan.scale_from_stats(st.ptr, 8, 0, 0, self.dXdP, self._dscale)
-        self.plan.scale(self.wignerfunction, self._dscale)
+        rho /= n (device-side, no host sync).  n is read from rho as Re <L^H 1, rho> dXdP (the
+        transform L is linear), one launch pair; W is then formed already divided by n (the
+        scale rides the transform's first pass) and rho is divided in one pass, instead of a
+        read of W and two scale passes after the transform."""
+        if not self.obs_by_adjoint:
+            self.rho2wigner()
+            st = self.plan.stats(self.wignerfunction)
+            self.plan.scale_from_stats(st.ptr, 8, 0, 0, self.dXdP, self._dscale)
+            self.plan.scale(self.wignerfunction, self._dscale)
+            self.plan.scale(self.rho, self._dscale)
+            return self
+        self._flush_scale()
+        g1 = self._norm_weights()
+        self.plan.weighted_sums(self.rho, g1, 1, self._norm_out, self.dXdP, self._dscale)
+        wm_ops.rho2wigner(self.plan, self._grid, self.rho, self.wignerfunction, dscale=self._dscale)
         self.plan.scale(self.rho, self._dscale)
         return self
 

@@ -69,10 +69,11 @@ def wigner2rho(plan, g, W, rho):
     shear_x(plan, g, rho, INV, FWD, sign_out=1)
 
 
-def rho2wigner(plan, g, rho, W):
+def rho2wigner(plan, g, rho, W, dscale=None):
     """wigner_moyal_cuda_1d.py:471-501: [S, F1, SX, I1] [F0, SY, I0] [F1, SX, I1] [I0, S]
-    (unnormalised, like the reference)."""
-    shear_x(plan, g, rho, FWD, INV, sign_in=1, out=W)
+    (unnormalised, like the reference; ``dscale``: complex per-item factors applied in the
+    first pass)."""
+    shear_x(plan, g, rho, FWD, INV, sign_in=1, out=W, dscale=dscale)
     shear_y(plan, g, W, FWD, INV)
     shear_x(plan, g, W, FWD, INV)
     plan.run(W, 0, INV, 0, sign_out=1)

@@ -207,3 +207,57 @@ def test_sw_ground_and_stationary_states():
     o.get_stationary_states(3, nsteps=600)
     for a, b in zip(g.stationary_states, o.stationary_states):
         assert rel(a.get(), b) < 1e-9
+
+
+def test_sw_wigner_blackman_and_closed_form():
+    """get_wigner against the coherent-state closed form exp(-(x-x0)^2 - (p-p0)^2) / (2 pi)
+    on (X_wigner, P_wigner), and get_wigner_blackman (schrodinger_wigner_cuda_1d.py:627-663)
+    against the oracle, FP64 and mixed."""
+    from accuratewigner_amd import SchrodingerWignerCUDA1D
+    x0, p0 = 1.3, -0.7
+    for prec, tol in (("fp64", 1e-12), ("mixed", 2e-6)):
+        g = SchrodingerWignerCUDA1D(X_gridDIM=256, X_amplitude=12., dt=0.01, t=0., V="0.5 * X * X",
+                                    K="0.5 * P * P", precision=prec)
+        o = SchrodingerWignerOracle(X_gridDIM=256, X_amplitude=12., dt=0.01, V=lambda X, t: 0.5 * X * X, K=HARM_K)
+        g.set_wavefunction("exp(-pow(X - 1.3, 2) / 2.) * cuda_complex(cos(-0.7 * X), sin(-0.7 * X))")
+        W = g.get_wigner().get().astype(np.complex128)
+        exact = np.exp(-(g.X_wigner[None, :] - x0) ** 2 - (g.P_wigner[:, None] - p0) ** 2) / (2 * np.pi)
+        assert np.max(np.abs(W - exact)) < (1e-11 if prec == "fp64" else 1e-6)
+        rng = np.random.default_rng(9)
+        psi = (rng.normal(size=256) + 1j * rng.normal(size=256)) * np.exp(-(g.X / 6.) ** 2)
+        g.set_wavefunction(psi.reshape(1, -1).astype(g.state_dtype))
+        o.set_wavefunction(psi.reshape(1, -1))
+        Wb = g.get_wigner_blackman().get().astype(np.complex128)
+        assert rel(Wb, o.get_wigner(blackman=True)) < tol
+        assert rel(g.get_wigner().get().astype(np.complex128), o.get_wigner()) < tol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_wigner_bloch_gibbs_config2_full_size(precision):
+    """Config 2 at its own size: V = x^2/2 + 0.1 x^4, 512^2, x, p in [-8, 8), beta 0 -> 2 in
+    d beta = 0.005 (400 steps), <H>(beta) every step.  Here the merged chunks' FP64 weights
+    are cut off where the opening Bloch factor C0 = exp(-d beta V / 4 ...) underflows below
+    1e-150 (|XX'| reaches 71 on this grid).  FP64: W and <H>(beta) within 1e-10 of the
+    oracle; mixed: within 1e-5 / 5e-5 of FP64."""
+    import os
+    from accuratewigner_amd import WignerBlochCUDA1D
+    from oracle.awig_oracle import DFT
+    V = "0.5 * X * X + 0.1 * X * X * X * X"
+    Vf = lambda X, t: 0.5 * X * X + 0.1 * X ** 4  # noqa: E731
+    kw = dict(X_gridDIM=512, X_amplitude=8., P_gridDIM=512, P_amplitude=8., kT=0.5, dbeta=0.005)
+    g = WignerBlochCUDA1D(V=V, K="0.5 * P * P", t=0., precision=precision, **kw)
+    W = g.get_gibbs_state(record=[g.hamiltonian]).get().astype(np.complex128)
+    assert g.gibbs_record.shape == (400, 1)
+    if precision == "fp64":
+        o = WignerBlochOracle(V=Vf, K=HARM_K, dft=DFT(min(os.cpu_count() or 1, 16)), **kw)
+        Ho = []
+        o.get_gibbs_state(callback=lambda k: Ho.append(o.get_average(lambda X, P, t: 0.5 * P * P + Vf(X, t))))
+        assert not o.warnings
+        assert rel(W, o.wignerfunction) < 1e-10
+        assert np.max(np.abs(g.gibbs_record[:, 0] - np.array(Ho))) < 1e-10
+        assert abs(g.get_purity() - o.get_purity()) < 1e-11
+    else:
+        g64 = WignerBlochCUDA1D(V=V, K="0.5 * P * P", t=0., **kw)
+        W64 = g64.get_gibbs_state(record=[g64.hamiltonian]).get()
+        assert rel(W, W64) < 1e-5
+        assert np.max(np.abs(g.gibbs_record - g64.gibbs_record) / np.abs(g64.gibbs_record)) < 5e-5

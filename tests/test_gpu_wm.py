@@ -64,6 +64,19 @@ def test_wigner_transforms_match_oracle():
     assert rel(g.wignerfunction.get(), o.wignerfunction) < 1e-12
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_wm_rho2wigner_blackman_matches_oracle(precision):
+    """rho2wigner_blackman (wigner_moyal_cuda_1d.py:503-538) on a propagated state."""
+    g, o = _pair(Nx=256, Np=128, Xa=8., Pa=6., precision=precision)
+    g.set_wignerfunction("exp(-pow(X - 1., 2) - pow(P - 0.5, 2)) / M_PI")
+    o.set_wignerfunction(lambda X, P: np.exp(-(X - 1.) ** 2 - (P - 0.5) ** 2) / np.pi)
+    g.propagate(10)
+    o.propagate(10)
+    Wb = g.rho2wigner_blackman().get().astype(np.complex128)
+    o.rho2wigner_(blackman=True)
+    assert rel(Wb, o.wignerfunction) < (1e-12 if precision == "fp64" else 2e-6)
+
+
 def test_wm_propagate_fp64_matches_oracle():
     V = "0.5 * X * X + 0.1 * X * X * X * X"
     g, o = _pair(V=V, Vf=lambda X, t: 0.5 * X * X + 0.1 * X ** 4)
