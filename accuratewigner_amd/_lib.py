@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libawig.so")
 AW_OK = 0
 AW_PREC_F64 = 0
 AW_PREC_F32 = 1
+AW_PREC_C64 = 2   # complex64 states, FP64 arithmetic
 AW_FORWARD = -1
 AW_INVERSE = 1
 

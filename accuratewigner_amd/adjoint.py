@@ -122,7 +122,7 @@ class WignerMoyalDipoleControl:
         through the segment with the adjoint steps, with the pair dots of each step (and of
         the segment's end state for the last segment) in the segment's dot rows."""
         s, p = self.sys, self.sys.plan
-        esz = self._esz
+        esz = p.fsize                     # factor vectors (the states use self._esz)
         Nx, Np = s.X_gridDIM, s.P_gridDIM
         colL, colE, rowL, rowE = self._segF[1]
         aL, aE, bL, bE = self._segF[-1]

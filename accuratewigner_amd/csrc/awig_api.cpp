@@ -37,6 +37,11 @@ struct aw_module {
 
 static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
+// precision of a plan's twiddles, factor vectors, tables and masks (its arithmetic)
+static int arith_prec(int prec) { return prec == AW_PREC_F32 ? AW_PREC_F32 : AW_PREC_F64; }
+// precision of the states it reads and writes
+static int state_prec(int prec) { return prec == AW_PREC_F64 ? AW_PREC_F64 : AW_PREC_F32; }
+
 // exp(-2 pi i t / n) in FP64 with exact octant reduction of t/n before calling libm
 static void host_root(long long t, long long n, double &c, double &s) {
     long long a = ((t % n) + n) % n;
@@ -81,7 +86,7 @@ const char *aw_last_error(void) { return g_err.c_str(); }
 int aw_plan_create(aw_plan **plan, int rows, int cols, int batch, int prec) {
     if (!plan) return set_error(AW_ERR_INVALID_VALUE, "plan pointer is NULL");
     *plan = nullptr;
-    if (prec != AW_PREC_F64 && prec != AW_PREC_F32)
+    if (prec != AW_PREC_F64 && prec != AW_PREC_F32 && prec != AW_PREC_C64)
         return set_error(AW_ERR_INVALID_TYPE, "unsupported precision %d", prec);
     if (batch < 1) return set_error(AW_ERR_INVALID_SIZE, "batch must be >= 1 (got %d)", batch);
     if (!is_pow2(cols) || cols < 64 || cols > 4096)
@@ -94,8 +99,8 @@ int aw_plan_create(aw_plan **plan, int rows, int cols, int batch, int prec) {
     p->batch = batch;
     p->prec = prec;
     p->csize = prec == AW_PREC_F64 ? 16 : 8;
-    int rc = make_twiddles(cols, prec, &p->tw_cols);
-    if (rc == AW_OK && rows > 1) rc = make_twiddles(rows, prec, &p->tw_rows);
+    int rc = make_twiddles(cols, arith_prec(prec), &p->tw_cols);
+    if (rc == AW_OK && rows > 1) rc = make_twiddles(rows, arith_prec(prec), &p->tw_rows);
     if (rc == AW_OK) {
         const long long n = (long long)rows * cols;
         long long nb = (n + 256 * 8 - 1) / (256 * 8);
@@ -256,7 +261,8 @@ int aw_step_sequence(aw_plan *p, void *data, const aw_steps *st, aw_stream strea
     if (p->rows < 2) return set_error(AW_ERR_INVALID_VALUE, "step sequences need a 2D plan");
     if (st->batch < 0 || st->batch > p->batch)
         return set_error(AW_ERR_INVALID_VALUE, "batch %d outside 0..%d", st->batch, p->batch);
-    const long long esz = p->csize;
+    const long long esz = p->csize;                                   // state elements
+    const long long fsz = arith_prec(p->prec) == AW_PREC_F64 ? 16 : 8;  // factor-vector elements
     const char *cl = static_cast<const char *>(st->col_line), *ce = static_cast<const char *>(st->col_elem);
     const char *rl = static_cast<const char *>(st->row_line), *re = static_cast<const char *>(st->row_elem);
     if (!cl || !ce || !rl || !re) return set_error(AW_ERR_INVALID_VALUE, "NULL factor table");
@@ -267,14 +273,14 @@ int aw_step_sequence(aw_plan *p, void *data, const aw_steps *st, aw_stream strea
     // column vector v of the sequence and row vector s (constant tables reuse 0/1/2 and 0)
     auto cvec = [&](int v) { return st->const_steps ? (v == 0 ? 0 : (v == n ? 2 : 1)) : v; };
     auto rvec = [&](int s) { return st->const_steps ? 0 : s; };
-    const long long rsz = esz / 2;   // masks are real
+    const long long rsz = fsz / 2;   // masks are real, in the arithmetic precision
     int rc = AW_OK;
     for (int b0 = 0; rc == AW_OK && b0 < active; b0 += group) {
         const int nb = std::min(group, active - b0);
         // per-item tables advance by their batch strides (0: shared by all items)
         char *d = static_cast<char *>(data) + b0 * item_elems * esz;
-        const char *gcl = cl + b0 * st->col_line_bstride * esz, *gce = ce + b0 * st->col_elem_bstride * esz;
-        const char *grl = rl + b0 * st->row_line_bstride * esz, *gre = re + b0 * st->row_elem_bstride * esz;
+        const char *gcl = cl + b0 * st->col_line_bstride * fsz, *gce = ce + b0 * st->col_elem_bstride * fsz;
+        const char *grl = rl + b0 * st->row_line_bstride * fsz, *gre = re + b0 * st->row_elem_bstride * fsz;
         auto moff = [&](const void *m) -> const void * {
             return m ? static_cast<const char *>(m) + b0 * st->mask_bstride * rsz : nullptr;
         };
@@ -292,15 +298,15 @@ int aw_step_sequence(aw_plan *p, void *data, const aw_steps *st, aw_stream strea
         ps.dscale = nullptr;
         for (int s = 0; rc == AW_OK && s < n; ++s) {
             ps.axis = 1; ps.dir_a = AW_FORWARD; ps.dir_b = AW_INVERSE;
-            ps.line_fac = grl + rvec(s) * p->rows * esz; ps.line_fac_bstride = st->row_line_bstride;
-            ps.elem_fac = gre + rvec(s) * p->cols * esz; ps.elem_fac_bstride = st->row_elem_bstride;
+            ps.line_fac = grl + rvec(s) * p->rows * fsz; ps.line_fac_bstride = st->row_line_bstride;
+            ps.elem_fac = gre + rvec(s) * p->cols * fsz; ps.elem_fac_bstride = st->row_elem_bstride;
             ps.mask = m_row;
             rc = pass_run_n(p, d, &ps, nb, stream);
             if (rc != AW_OK) break;
             const bool last = s == n - 1;
             ps.axis = 0; ps.dir_a = AW_FORWARD; ps.dir_b = last ? 0 : AW_INVERSE;
-            ps.line_fac = gcl + cvec(s + 1) * p->cols * esz; ps.line_fac_bstride = st->col_line_bstride;
-            ps.elem_fac = gce + cvec(s + 1) * p->rows * esz; ps.elem_fac_bstride = st->col_elem_bstride;
+            ps.line_fac = gcl + cvec(s + 1) * p->cols * fsz; ps.line_fac_bstride = st->col_line_bstride;
+            ps.elem_fac = gce + cvec(s + 1) * p->rows * fsz; ps.elem_fac_bstride = st->col_elem_bstride;
             ps.mask = last ? m_oc : m_mid;
             rc = pass_run_n(p, d, &ps, nb, stream);
         }
@@ -340,7 +346,7 @@ int aw_make_factors(aw_plan *p, void *out, long long out_stride, int n, int nvec
     f.mag = mag; f.mag_stride = mag_stride; f.mag_pow = mag_pow;
     f.scale = scale;
     f.alt_sign = alt_sign;
-    return launch_make_factors(p->prec, f, (hipStream_t)stream);
+    return launch_make_factors(arith_prec(p->prec), f, (hipStream_t)stream);
 }
 
 int aw_reduce_stats(aw_plan *p, const void *data, double *out, aw_stream stream) {
@@ -462,7 +468,8 @@ int aw_sum_partials(const double *partials, int nblk, int ncomp, int batch, doub
 
 int aw_reduce_vec(int prec, const void *data, long long n, int batch, double *out, aw_stream stream) {
     if (!data || !out || n < 1 || batch < 1) return set_error(AW_ERR_INVALID_VALUE, "bad vector reduction");
-    if (prec != AW_PREC_F64 && prec != AW_PREC_F32) return set_error(AW_ERR_INVALID_TYPE, "bad precision");
+    if (prec != AW_PREC_F64 && prec != AW_PREC_F32 && prec != AW_PREC_C64)
+        return set_error(AW_ERR_INVALID_TYPE, "bad precision");
     // one block per item in a fixed order, no scratch (nothing is allocated; a plan's
     // aw_reduce_stats splits large items over many blocks)
     return launch_stats(prec, data, n, 1, (int)(n > 0x7fffffff ? 0x7fffffff : n), batch, nullptr, 1, out,
@@ -484,7 +491,10 @@ int aw_scale(int prec, void *data, long long n, int batch, const double *dscale,
 
 int aw_convert(const void *src, int src_prec, void *dst, int dst_prec, long long n, aw_stream stream) {
     if (!src || !dst) return set_error(AW_ERR_INVALID_VALUE, "NULL pointer");
-    return launch_convert(src, src_prec, dst, dst_prec, n, (hipStream_t)stream);
+    for (int q : {src_prec, dst_prec})
+        if (q != AW_PREC_F64 && q != AW_PREC_F32 && q != AW_PREC_C64)
+            return set_error(AW_ERR_INVALID_TYPE, "bad precision %d", q);
+    return launch_convert(src, state_prec(src_prec), dst, state_prec(dst_prec), n, (hipStream_t)stream);
 }
 
 int aw_project_out(int prec, void *psi, const void *phis, int nphi, int n, aw_stream stream) {

@@ -44,11 +44,14 @@ int set_error(int code, const char *fmt, ...) {
 // ---------------------------------------------------------------------------------------
 int launch_pass_f32(int n, bool strided, int kind, const PassArgs &a, hipStream_t st);
 int launch_pass_f64(int n, bool strided, int kind, const PassArgs &a, hipStream_t st);
+int launch_pass_c64(int n, bool strided, int kind, const PassArgs &a, hipStream_t st);
 int obs_lpb_f32(int n, bool strided, long long total_lines);
 int obs_lpb_f64(int n, bool strided, long long total_lines);
 
+// the complex64-storage passes compute in FP64 with the FP64 tiling, so their blocks (and
+// partial-sum layouts) are those of the FP64 passes
 int obs_lpb(int prec, int n, bool strided, long long total_lines) {
-    return prec == AW_PREC_F64 ? obs_lpb_f64(n, strided, total_lines) : obs_lpb_f32(n, strided, total_lines);
+    return prec == AW_PREC_F32 ? obs_lpb_f32(n, strided, total_lines) : obs_lpb_f64(n, strided, total_lines);
 }
 
 int launch_pass(int prec, int n, bool strided, const PassArgs &a, hipStream_t st) {
@@ -73,6 +76,7 @@ int launch_pass(int prec, int n, bool strided, const PassArgs &a, hipStream_t st
             kind = kind == 2 ? 5 : 6;
         }
     }
+    if (prec == AW_PREC_C64) return launch_pass_c64(n, strided, kind, a, st);
     return prec == AW_PREC_F64 ? launch_pass_f64(n, strided, kind, a, st)
                                : launch_pass_f32(n, strided, kind, a, st);
 }

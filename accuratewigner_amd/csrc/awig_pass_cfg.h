@@ -96,14 +96,14 @@ template <typename R, int N, bool STRIDED> struct SmallTune {
 // Column exchanges in real and imaginary halves pad one row per 32 positions (PADC 5)
 // instead of 16: +2 % on 1024^2 x 96 mixed (8: +0.7 %, 64 and 128: slower); the row pad
 // stays at 32 positions (16 is 3 % slower, 64 no faster).
-template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0>
+template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0, typename S = R>
 using PassK = KCfg<R, N, PassTune<R, N, STRIDED>::E, PassTune<R, N, STRIDED>::LPB, STRIDED,
                    PassTune<R, N, STRIDED>::TWM, DA, DB, GEN, 1, PassTune<R, N, STRIDED>::PF,
-                   (PassTune<R, N, STRIDED>::XS && STRIDED) ? 5 : 4, 5, OBS, PassTune<R, N, STRIDED>::XS>;
+                   (PassTune<R, N, STRIDED>::XS && STRIDED) ? 5 : 4, 5, OBS, PassTune<R, N, STRIDED>::XS, S>;
 
-template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0>
+template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0, typename S = R>
 using PassKS = KCfg<R, N, SmallTune<R, N, STRIDED>::E, SmallTune<R, N, STRIDED>::LPB, STRIDED, 1, DA,
-                    DB, GEN, 1, GEN ? 0 : SmallTune<R, N, STRIDED>::PF, 4, 5, OBS>;
+                    DB, GEN, 1, GEN ? 0 : SmallTune<R, N, STRIDED>::PF, 4, 5, OBS, 0, S>;
 
 template <typename C>
 int launch_cfg(const PassArgs &a, hipStream_t st) {
@@ -124,44 +124,44 @@ int launch_cfg(const PassArgs &a, hipStream_t st) {
 //       5 / 6 column step [forward, factor, inverse] / [forward, factor] that stores the block
 //         partials of the values right after the factor multiply
 // (the partial kinds need blocks of whole waves: obs_lpb_n reports 0 where they are not)
-template <typename R, int N, bool STRIDED>
+template <typename R, int N, bool STRIDED, typename S = R>
 int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
     if constexpr (N <= 2048) {
         if (a.total_lines <= small_lines_max<R, N>()) {
             switch (kind) {
             case 4:
-                if constexpr (!STRIDED) return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0, 1>>(a, st);
+                if constexpr (!STRIDED) return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0, 1, S>>(a, st);
                 return set_error(AW_ERR_INVALID_VALUE, "weighted-sum partials need a row pass");
             case 5:
             case 6:
                 if constexpr (STRIDED && PassKS<R, N, STRIDED, -1, 1, 0>::THREADS % 64 == 0) {
-                    if (kind == 5) return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0, 2>>(a, st);
-                    return launch_cfg<PassKS<R, N, STRIDED, -1, 0, 0, 2>>(a, st);
+                    if (kind == 5) return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0, 2, S>>(a, st);
+                    return launch_cfg<PassKS<R, N, STRIDED, -1, 0, 0, 2, S>>(a, st);
                 }
                 return set_error(AW_ERR_INVALID_VALUE,
                                  "partials after the factor: column passes of whole waves only");
-            case 1: return launch_cfg<PassKS<R, N, STRIDED, 0, 1, 0>>(a, st);
-            case 2: return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0>>(a, st);
-            case 3: return launch_cfg<PassKS<R, N, STRIDED, -1, 0, 0>>(a, st);
-            default: return launch_cfg<PassKS<R, N, STRIDED, 2, 2, 1>>(a, st);
+            case 1: return launch_cfg<PassKS<R, N, STRIDED, 0, 1, 0, 0, S>>(a, st);
+            case 2: return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0, 0, S>>(a, st);
+            case 3: return launch_cfg<PassKS<R, N, STRIDED, -1, 0, 0, 0, S>>(a, st);
+            default: return launch_cfg<PassKS<R, N, STRIDED, 2, 2, 1, 0, S>>(a, st);
             }
         }
     }
     switch (kind) {
     case 4:
-        if constexpr (!STRIDED) return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0, 1>>(a, st);
+        if constexpr (!STRIDED) return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0, 1, S>>(a, st);
         return set_error(AW_ERR_INVALID_VALUE, "weighted-sum partials need a row pass");
     case 5:
     case 6:
         if constexpr (STRIDED && PassK<R, N, STRIDED, -1, 1, 0>::THREADS % 64 == 0) {
-            if (kind == 5) return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0, 2>>(a, st);
-            return launch_cfg<PassK<R, N, STRIDED, -1, 0, 0, 2>>(a, st);
+            if (kind == 5) return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0, 2, S>>(a, st);
+            return launch_cfg<PassK<R, N, STRIDED, -1, 0, 0, 2, S>>(a, st);
         }
         return set_error(AW_ERR_INVALID_VALUE, "partials after the factor: column passes of whole waves only");
-    case 1: return launch_cfg<PassK<R, N, STRIDED, 0, 1, 0>>(a, st);
-    case 2: return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0>>(a, st);
-    case 3: return launch_cfg<PassK<R, N, STRIDED, -1, 0, 0>>(a, st);
-    default: return launch_cfg<PassK<R, N, STRIDED, 2, 2, 1>>(a, st);
+    case 1: return launch_cfg<PassK<R, N, STRIDED, 0, 1, 0, 0, S>>(a, st);
+    case 2: return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0, 0, S>>(a, st);
+    case 3: return launch_cfg<PassK<R, N, STRIDED, -1, 0, 0, 0, S>>(a, st);
+    default: return launch_cfg<PassK<R, N, STRIDED, 2, 2, 1, 0, S>>(a, st);
     }
 }
 
@@ -188,10 +188,10 @@ template <typename R> int obs_lpb(int n, bool strided, long long total_lines) {
 #undef AW_CASE
 }
 
-template <typename R>
+template <typename R, typename S = R>
 int launch_pass_prec(int n, bool strided, int kind, const PassArgs &a, hipStream_t st) {
 #define AW_CASE(NN) \
-    case NN: return strided ? launch_kind_n<R, NN, true>(kind, a, st) : launch_kind_n<R, NN, false>(kind, a, st);
+    case NN: return strided ? launch_kind_n<R, NN, true, S>(kind, a, st) : launch_kind_n<R, NN, false, S>(kind, a, st);
     switch (n) {
         AW_CASE(64) AW_CASE(128) AW_CASE(256) AW_CASE(512) AW_CASE(1024) AW_CASE(2048) AW_CASE(4096)
     default: return set_error(AW_ERR_INVALID_SIZE, "unsupported line length %d (64..4096, power of two)", n);

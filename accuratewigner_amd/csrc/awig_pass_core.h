@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "awig_internal.h"
 
 namespace awig {
@@ -40,6 +42,17 @@ template <typename T2> __device__ __forceinline__ T2 cmul_conj(T2 a, T2 b) {
 }
 template <typename T2, typename S> __device__ __forceinline__ T2 cscale(T2 a, S s) {
     T2 r; r.x = a.x * s; r.y = a.y * s; return r;
+}
+// complex value in another precision (the identity when the types agree)
+template <typename D2, typename S2> __device__ __forceinline__ D2 cvt2(S2 a) {
+    if constexpr (std::is_same<D2, S2>::value) {
+        return a;
+    } else {
+        D2 r;
+        r.x = static_cast<decltype(r.x)>(a.x);
+        r.y = static_cast<decltype(r.y)>(a.y);
+        return r;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -258,9 +271,11 @@ __device__ __forceinline__ void fft_line(int dir, T2 *v, T2 *L, int l, int u, co
 // passes (GEN = 0) only multiply by line x element factor vectors (and an optional mask),
 // which keeps their code to two FFT bodies.
 template <typename R_, int N_, int E_, int LPB_, bool STRIDED_, int TWM_, int DA_ = 2, int DB_ = 2,
-          int GEN_ = 1, int MINW_ = 1, int PF_ = 0, int PADC_ = 4, int PADR_ = 5, int OBS_ = 0, int XS_ = 0>
+          int GEN_ = 1, int MINW_ = 1, int PF_ = 0, int PADC_ = 4, int PADR_ = 5, int OBS_ = 0, int XS_ = 0,
+          typename S_ = R_>
 struct KCfg {
-    using R = R_;
+    using R = R_;   // arithmetic, twiddles, factors and masks
+    using S = S_;   // the state in HBM (converted on load and store)
     static constexpr int N = N_, E = E_, LPB = LPB_, TWM = TWM_, DA = DA_, DB = DB_, GEN = GEN_;
     static constexpr int OBS = OBS_;     // weighted-sum block partials of the output (1) or of the
                                          // values right after the factor multiply (2)
@@ -328,6 +343,37 @@ __device__ __forceinline__ void obs_block_partials(const PassArgs &a, const T2 *
     }
 }
 
+// Multiply the thread's elements v[k] (element e = u + T k of its line) by the chirp
+// exp(i c lcoord ecoord(e)), ecoord(e) = estep (q - ecenter) with q = e, or its FFT-ordered
+// index (e + N/2) % N.  The phase is linear in k between the wrap of the FFT order, which
+// falls at k = E/2, so it is evaluated in FP64 with one sincos every R elements and rotated
+// by exp(i c lcoord estep T) in between (FP64 complex products: a relative error of a few
+// 1e-16 after R - 1 rotations), instead of a sincos per element.
+template <typename C, typename T2>
+__device__ __forceinline__ void apply_chirp(T2 *v, double c, double lcoord, const PassArgs &a, int u) {
+    constexpr int N = C::N, E = C::E, T = C::T;
+    constexpr int R = (E >= 16) ? 8 : (E >= 2 ? E / 2 : 1);
+    const double cl = c * lcoord;
+    double sd, cd;
+    sincos(cl * (a.estep * double(T)), &sd, &cd);
+#pragma unroll
+    for (int k0 = 0; k0 < E; k0 += R) {
+        const int e = u + T * k0;
+        const int q = a.efft ? ((e + N / 2) % N) : e;
+        double sn, cs;
+        sincos(c * (lcoord * (a.estep * (double(q) - a.ecenter))), &sn, &cs);
+#pragma unroll
+        for (int k = k0; k < k0 + R; ++k) {
+            v[k] = cmul(v[k], mk<T2>(cs, sn));
+            if (k + 1 < k0 + R) {
+                const double c2 = cs * cd - sn * sd;
+                sn = cs * sd + sn * cd;
+                cs = c2;
+            }
+        }
+    }
+}
+
 template <typename C>
 __global__ __launch_bounds__(C::THREADS) __attribute__((amdgpu_waves_per_eu(C::MINW)))
 void k_line_pass(PassArgs a) {
@@ -384,10 +430,11 @@ void k_line_pass(PassArgs a) {
     const long long item = b * (long long)N * nl;
     const int off0 = STRIDED ? (u * nl + line) : (line * N + u);
     const int kstep = STRIDED ? T * nl : T;
-    const T2 *src = reinterpret_cast<const T2 *>(a.in) + item + off0;
+    using S2 = typename Cx<typename C::S>::T;
+    const S2 *src = reinterpret_cast<const S2 *>(a.in) + item + off0;
     T2 v[E];
 #pragma unroll
-    for (int k = 0; k < E; ++k) v[k] = src[k * kstep];
+    for (int k = 0; k < E; ++k) v[k] = cvt2<T2>(src[k * kstep]);
     // optional prefetch of the element factors: their load latency then overlaps the data
     // loads instead of stalling between the two FFTs (costs 2E registers)
     T2 efp[C::PF ? E : 1];
@@ -456,16 +503,7 @@ void k_line_pass(PassArgs a) {
         }
         if (C::GEN && a.chirp != 0.0) {
             const int ql = a.lfft ? ((line + nl / 2) % nl) : line;
-            const double lcoord = a.lstep * (double(ql) - a.lcenter);
-#pragma unroll
-            for (int k = 0; k < E; ++k) {
-                const int e = u + T * k;
-                const int q = a.efft ? ((e + N / 2) % N) : e;
-                const double ph = a.chirp * (lcoord * (a.estep * (double(q) - a.ecenter)));
-                double sn, cs;
-                sincos(ph, &sn, &cs);
-                v[k] = cmul(v[k], mk<T2>(cs, sn));
-            }
+            apply_chirp<C>(v, a.chirp, a.lstep * (double(ql) - a.lcenter), a, u);
         }
     }
 
@@ -487,16 +525,7 @@ void k_line_pass(PassArgs a) {
     if (C::GEN && a.chirp_out != 0.0) {
         // a second chirp after the second FFT (the twiddles of a two-level transform)
         const int ql = a.lfft ? ((line + nl / 2) % nl) : line;
-        const double lcoord = a.lstep * (double(ql) - a.lcenter);
-#pragma unroll
-        for (int k = 0; k < E; ++k) {
-            const int e = u + T * k;
-            const int q = a.efft ? ((e + N / 2) % N) : e;
-            const double ph = a.chirp_out * (lcoord * (a.estep * (double(q) - a.ecenter)));
-            double sn, cs;
-            sincos(ph, &sn, &cs);
-            v[k] = cmul(v[k], mk<T2>(cs, sn));
-        }
+        apply_chirp<C>(v, a.chirp_out, a.lstep * (double(ql) - a.lcenter), a, u);
     }
     if (C::GEN && a.sign_out) {
 #pragma unroll
@@ -510,9 +539,9 @@ void k_line_pass(PassArgs a) {
     if constexpr (C::OBS == 1) obs_block_partials<C>(a, v, active, off0, kstep, blk, nl, tid);
     if (!active) return;
     if (!C::GEN || a.out_mode == 0) {
-        T2 *dst = reinterpret_cast<T2 *>(a.out) + item + off0;
+        S2 *dst = reinterpret_cast<S2 *>(a.out) + item + off0;
 #pragma unroll
-        for (int k = 0; k < E; ++k) dst[k * kstep] = v[k];
+        for (int k = 0; k < E; ++k) dst[k * kstep] = cvt2<S2>(v[k]);
     } else {
         // keep only the element on the diagonal (e == line) of this line
         if ((line % T) == u && line < N) {
@@ -521,7 +550,7 @@ void k_line_pass(PassArgs a) {
 #pragma unroll
             for (int k = 0; k < E; ++k)
                 if (k == kk) val = v[k];
-            reinterpret_cast<T2 *>(a.out)[b * nl + line] = val;
+            reinterpret_cast<S2 *>(a.out)[b * nl + line] = cvt2<S2>(val);
         }
     }
 }

@@ -11,9 +11,15 @@ import numpy as np
 from . import _lib
 from .device import DeviceArray, current_stream, empty
 
-PRECISIONS = {"fp64": _lib.AW_PREC_F64, "mixed": _lib.AW_PREC_F32, "fp32": _lib.AW_PREC_F32}
-STATE_DTYPE = {_lib.AW_PREC_F64: np.complex128, _lib.AW_PREC_F32: np.complex64}
-REAL_DTYPE = {_lib.AW_PREC_F64: np.float64, _lib.AW_PREC_F32: np.float32}
+# "fp64": complex128 states and FP64 arithmetic (the reference's); "mixed": complex64 states
+# and FP32 FFT / factor arithmetic; "c64": complex64 states with FP64 twiddles, factors, masks
+# and arithmetic (only the stored state is rounded to FP32)
+PRECISIONS = {"fp64": _lib.AW_PREC_F64, "mixed": _lib.AW_PREC_F32, "fp32": _lib.AW_PREC_F32,
+              "c64": _lib.AW_PREC_C64}
+STATE_DTYPE = {_lib.AW_PREC_F64: np.complex128, _lib.AW_PREC_F32: np.complex64, _lib.AW_PREC_C64: np.complex64}
+# factor vectors, complex tables and (as the real dtype) masks follow the arithmetic
+FACTOR_DTYPE = {_lib.AW_PREC_F64: np.complex128, _lib.AW_PREC_F32: np.complex64, _lib.AW_PREC_C64: np.complex128}
+REAL_DTYPE = {_lib.AW_PREC_F64: np.float64, _lib.AW_PREC_F32: np.float32, _lib.AW_PREC_C64: np.float64}
 
 FWD, INV = _lib.AW_FORWARD, _lib.AW_INVERSE
 
@@ -52,7 +58,9 @@ class Plan:
         self.rows, self.cols, self.batch = int(rows), int(cols), int(batch)
         self.prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         self.state_dtype = STATE_DTYPE[self.prec]
+        self.factor_dtype = FACTOR_DTYPE[self.prec]
         self.real_dtype = REAL_DTYPE[self.prec]
+        self.fsize = np.dtype(self.factor_dtype).itemsize   # bytes per factor-vector element
         h = ctypes.c_void_p()
         _lib.call("aw_plan_create", ctypes.byref(h), self.rows, self.cols, self.batch, self.prec)
         self.handle = h
@@ -131,7 +139,7 @@ class Plan:
         """out[out_offset + v * out_vstride][k] = scale * (-1)^k? * mag^pow * E(c0 + ca A + cb B),
         E = exp(i.) (mode 0) or exp(.) (mode 1), in units of length-n vectors; A, B, mag are
         FP64 device pointers (ints) or None."""
-        esz = np.dtype(self.state_dtype).itemsize
+        esz = self.fsize
         _lib.call("aw_make_factors", self.handle, ctypes.c_void_p(_p(out) + out_offset * n * esz),
                   out_vstride * n, n, nvec, mode,
                   ctypes.c_void_p(A), n if a_stride is None else a_stride, ca,
@@ -140,7 +148,7 @@ class Plan:
                   scale, int(bool(alt_sign)), ctypes.c_void_p(current_stream()))
 
     def new_factors(self, n, nvec):
-        return empty((nvec, n), self.state_dtype)
+        return empty((nvec, n), self.factor_dtype)
 
     # -- reductions --------------------------------------------------------------------
     def stats(self, data, out=None):

@@ -77,7 +77,7 @@ class RhoBlochCUDA1D(RhoVNeumannCUDA1D):
 
     def _bloch_passes(self, dbeta):
         col, row = self._bloch_factors(dbeta)
-        csz = np.dtype(self.state_dtype).itemsize
+        csz = self.plan.fsize
         N, p, rho = self.X_gridDIM, self.plan, self.rho
         p.run(rho, 0, 0, INV, line_fac=col.ptr, elem_fac=col.ptr,
               dscale=self._dscale if self._scale_pending else None)

@@ -288,7 +288,7 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
         """Steps at times ts as 2n+1 fused passes."""
         n = len(ts)
         N = self.X_gridDIM
-        csz = np.dtype(self.state_dtype).itemsize
+        csz = self.plan.fsize
         colL, colE, rowL, rowE = self._chunk_factors(ts)
         p, rho = self.plan, self.rho
         ev = self._pass_events
@@ -310,7 +310,7 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
         and the next scale 1 / (Re tr(rho) dX).  Only launches, from fixed addresses."""
         bufs, p, rho = self._ehr, self.plan, self.rho
         N = self.X_gridDIM
-        csz = np.dtype(self.state_dtype).itemsize
+        csz = self.plan.fsize
         colL, colE, rowL, rowE = bufs["fac"]
         stage, xw, pwc = bufs["stage"], bufs["xw"], bufs["pwc"]
         for k in range(n):

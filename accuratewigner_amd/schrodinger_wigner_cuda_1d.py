@@ -376,7 +376,7 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
         """Steps at times ts; merged half-kicks unless ``per_step`` needs psi after each step."""
         n = len(ts)
         N = self.X_gridDIM
-        csz = np.dtype(self.state_dtype).itemsize
+        csz = self.plan.fsize
         p, psi = self.plan, self.wavefunction
         if self._two is not None:
             self._run_two(ts, bloch, per_step, raw)
@@ -403,7 +403,7 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
         """_run for two-level lines: EV and EK are full-line tables applied inside the passes."""
         n = len(ts)
         N = self.X_gridDIM
-        csz = np.dtype(self.state_dtype).itemsize
+        csz = self.plan.fsize
         psi = self.wavefunction
         if per_step is None:
             ev, ek = self._factors(ts, bloch, raw)
@@ -423,7 +423,7 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
     def _two_step(self, ev, ek):
         """One unmerged two-level step (three passes) from factor tables ev[0..1], ek[0]."""
         N, psi = self.X_gridDIM, self.wavefunction
-        csz = np.dtype(self.state_dtype).itemsize
+        csz = self.plan.fsize
         self._col(psi, 0, FWD, ctab=ev.ptr, twiddle=True, dscale=self._dscale if self._scale_pending else None)
         self._scale_pending = False
         self._row(psi, FWD, INV, ctab=ek.ptr, twiddle=True)
@@ -531,7 +531,7 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
         even = True
         t0 = np.full(1, float(self.t_initial))
         ev, ek = self._factors(t0, bloch=True, raw=True)
-        csz = np.dtype(self.state_dtype).itemsize
+        csz = self.plan.fsize
         p, psi = self.plan, self.wavefunction
         for n in range(nstates):
             if even:
@@ -591,7 +591,7 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
         if blackman:
             if self._blackman is None:
                 c = np.arange(N) * 2.0 * np.pi / (N - 1.)
-                self._blackman = to_device((0.42 - 0.5 * np.cos(c) + 0.08 * np.cos(2.0 * c)).astype(self.state_dtype))
+                self._blackman = to_device((0.42 - 0.5 * np.cos(c) + 0.08 * np.cos(2.0 * c)).astype(self.plan.factor_dtype))
             win = self._blackman
             wp.run(W, 1, 0, 0, elem_fac=win)
             wp.run(W, 1, FWD, INV, **shx)

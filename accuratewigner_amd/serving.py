@@ -133,6 +133,7 @@ class _Pool:
             self.plan = None
             self.plan1 = Plan(N, N, 1, precision)
             self.dtype = self.plan1.state_dtype
+            self.fdtype = self.plan1.factor_dtype    # factor tables follow the arithmetic
             self.state_t = "double2" if self.dtype == np.complex128 else "float2"
             self.mod = _serve_module(self.state_t)
             self.g = WMGrid(N, L, N, L)
@@ -162,7 +163,8 @@ class _Pool:
     def slot_cost(N, precision):
         """HBM bytes per slot: the state and its 8 factor vectors."""
         esz = 16 if precision == "fp64" else 8
-        return N * N * esz + 8 * N * esz
+        fsz = 8 if precision == "mixed" else 16
+        return N * N * esz + 8 * N * fsz
 
     @staticmethod
     def fixed_cost(N, precision, kind):
@@ -185,8 +187,8 @@ class _Pool:
         N, n = self.N, len(self.jobs)
         with torch.cuda.stream(self.stream):
             state = DeviceArray((capacity, N, N), self.dtype)
-            tabs = [empty((capacity, 3, N), self.dtype), empty((capacity, 3, N), self.dtype),
-                    empty((capacity, 1, N), self.dtype), empty((capacity, 1, N), self.dtype)]
+            tabs = [empty((capacity, 3, N), self.fdtype), empty((capacity, 3, N), self.fdtype),
+                    empty((capacity, 1, N), self.fdtype), empty((capacity, 1, N), self.fdtype)]
             if n:
                 state.tensor[:n].copy_(self.state.tensor[:n])
                 for new, old in zip(tabs, (self.colL, self.colE, self.rowL, self.rowE)):
@@ -199,7 +201,7 @@ class _Pool:
     def _factors(self, k, job):
         """Per-slot tables of the merged Strang step (constant-step layout 0 open, 1 mid, 2 close)."""
         N, p = self.N, self.plan
-        esz = np.dtype(self.dtype).itemsize
+        esz = p.fsize
         for q, (coord, which) in enumerate(zip(self.coords, (0, 0, 1, 1))):
             self.mod.launch("aw_serve_vk", (max(1, N // 256),), (256,),
                             [coord.ptr, ctypes.c_int(N), ctypes.c_double(job.c4), ctypes.c_int(which),

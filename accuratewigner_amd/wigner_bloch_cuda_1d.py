@@ -103,7 +103,7 @@ class WignerBlochCUDA1D(WignerMoyalCUDA1D):
 
     def _bloch_step_passes(self, dbeta):
         colL, colE, rowL, rowE = self._bloch_factors(dbeta)
-        csz = np.dtype(self.state_dtype).itemsize
+        csz = self.plan.fsize
         Nx, Np = self.X_gridDIM, self.P_gridDIM
         p, rho = self.plan, self.rho
         p.run(rho, 0, 0, INV, line_fac=colL.ptr, elem_fac=colE.ptr,
@@ -198,7 +198,7 @@ class WignerBlochCUDA1D(WignerMoyalCUDA1D):
             cl = np.exp(q * (vx - self.V_min_x)) * (1.0 - 2.0 * (np.arange(Nx) & 1))
             ce = np.exp(q * (vxp - self.V_min_xp)) * (1.0 - 2.0 * (np.arange(Np) & 1))
             c0 = np.outer(ce, cl)
-            keep = np.abs(c0) > (1e-150 if self.precision == "fp64" else 1e-17)
+            keep = np.abs(c0) > (1e-17 if self.precision == "mixed" else 1e-150)
             inv = np.where(keep, 1.0 / np.where(keep, c0, 1.0), 0.0)
             g = self._gibbs_weights(record)
             colLm, colEm = p.new_factors(Nx, 1), p.new_factors(Np, 1)
@@ -215,7 +215,7 @@ class WignerBlochCUDA1D(WignerMoyalCUDA1D):
         sum |rho|^2] of the unnormalised states go to fb["stats"][:n], and the last step's
         normalisation scale to self._dscale (applied by the next pass that opens a step)."""
         colL, colE, rowL, rowE = self._bloch_factors(dbeta)
-        csz = np.dtype(self.state_dtype).itemsize
+        csz = self.plan.fsize
         Nx, Np, B = self.X_gridDIM, self.P_gridDIM, self.batch
         p, rho = self.plan, self.rho
         nblk, nc = mw["nblk"], mw["nc"]

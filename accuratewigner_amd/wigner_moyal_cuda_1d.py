@@ -450,7 +450,7 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
     def _blackman(self, n):
         """blackmanX / blackmanY (:678-698) as a complex vector in the state precision."""
         c = np.arange(n) * 2.0 * np.pi / (n - 1.)
-        return to_device((0.42 - 0.5 * np.cos(c) + 0.08 * np.cos(2.0 * c)).astype(self.state_dtype))
+        return to_device((0.42 - 0.5 * np.cos(c) + 0.08 * np.cos(2.0 * c)).astype(self.plan.factor_dtype))
 
     def _flush_scale(self):
         """Apply a pending device-side normalisation to rho."""
@@ -595,7 +595,7 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         n = len(ts)
         Nx, Np = self.X_gridDIM, self.P_gridDIM
         colL, colE, rowL, rowE = self._build_chunk_factors(ts)
-        csz = np.dtype(self.state_dtype).itemsize
+        csz = self.plan.fsize
         _, _, tab1 = self._mask_args(0, 1)
         _, _, tab2 = self._mask_args(0, 2)
         _, _, tabp = self._mask_args(1, 1)
@@ -722,7 +722,7 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         colL, colE, rowL, rowE = bufs["fac"]
         _, _, tab1 = self._mask_args(0, 1)
         _, _, tabp = self._mask_args(1, 1)
-        csz = np.dtype(self.state_dtype).itemsize
+        csz = self.plan.fsize
         Nx, Np = self.X_gridDIM, self.P_gridDIM
         nobs = len(exprs)
         row_bytes = self.batch * nobs * 8
@@ -754,7 +754,7 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         _, _, tab1 = self._mask_args(0, 1)
         _, _, tab2 = self._mask_args(0, 2)
         _, _, tabp = self._mask_args(1, 1)
-        csz = np.dtype(self.state_dtype).itemsize
+        csz = self.plan.fsize
         Nx, Np = self.X_gridDIM, self.P_gridDIM
         nobs, nblk = len(exprs), bufs["wnblk"]
         gy, ring = bufs["weights"], bufs["wring"]

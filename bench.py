@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "grid-point-steps/s vs plain FP64 at stated error; p50/p99 solve latency"
 HBM_PEAK_GBPS = 8000.0
+HEADLINE_PRECISION = "mixed"
 
 
 def make_system(N, batch, precision, L=10.0, dt=0.01):
@@ -121,63 +122,66 @@ def serving_run(n_jobs, seed=11, chunk=64):
             "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max())}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--grid", type=int, default=1024)
-    ap.add_argument("--batch", type=int, default=96)
-    ap.add_argument("--fp64-batch", type=int, default=48)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-jobs", type=int, default=200)
-    ap.add_argument("--copy-only", type=int, default=0, metavar="MIB",
-                    help="only run the copy-bandwidth kernel on MIB MiB (counter calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the propagators would shard over independent states, one batch slice per GPU)", file=sys.stderr)
-        sys.exit(2)
-    import torch
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    N, B, K, W = args.grid, args.batch, args.steps, args.warmup
-    if args.copy_only:
-        gbps, ms = copy_bandwidth(args.copy_only, reps=3)
-        return {"copy_MiB": args.copy_only, "copy_GBps": gbps, "ms": ms}
-    copy_gbps, _ = copy_bandwidth()
+def host_info():
+    """CPU model, thread count and library versions of the host the CPU baseline runs on."""
+    import platform
+    import scipy
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "host_cpus": os.cpu_count(), "numpy": np.__version__, "scipy": scipy.__version__}
 
-    # -- headline: mixed precision, batch B -------------------------------------------------
-    mixed = make_system(N, B, "mixed")
+
+def leg_tier(N, B, K, W, precision):
+    """One complex64-state tier on the WM batch: throughput and the roofline of its step
+    launches (one read and one write of the complex64 states per pass)."""
+    mixed = make_system(N, B, precision)
     W0 = initial_states(mixed, B)
     mixed.set_wignerfunction(W0)
     wall, pass_ms = time_propagate(mixed, W, K)
-    value = B * N * N * K / wall
     # a pass over the batch is one launch per group of items (groups stay in the MALL
     # between passes; equal groups here): per launch, bytes and time are both 1/launches
     group = mixed.plan.step_group(min(K, 256), B)
-    launches = -(-B // group)
-    bytes_per_pass = 2 * B * N * N * 8          # one read + one write of the c64 state
-    bytes_per_launch = 2 * group * N * N * 8
     launch_ms = pass_ms * group / B
-    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    bytes_per_launch = 2 * group * N * N * 8      # one read + one write of the group's c64 states
+    return dict(sys=mixed, W0=W0, wall=wall, value=B * N * N * K / wall, pass_ms=pass_ms, group=group,
+                launches=-(-B // group), launch_ms=launch_ms, bytes_per_launch=bytes_per_launch,
+                bytes_per_pass=2 * B * N * N * 8,
+                achieved=bytes_per_launch / (launch_ms * 1e-3) / 1e9)
 
-    # -- plain FP64 path on the first Bf states ---------------------------------------------
-    Bf = min(args.fp64_batch, B)
+
+def rel_err(sysm, ref, Bf):
+    """Largest per-state relative L2 error of the first Bf states of sysm against ref."""
+    a = sysm.wignerfunction.get()[:Bf].astype(np.complex128)
+    return float(np.max(np.linalg.norm((a - ref).reshape(Bf, -1), axis=1)
+                        / np.linalg.norm(ref.reshape(Bf, -1), axis=1)))
+
+
+def leg_fp64(N, Bf, K, W, W0, tiers):
+    """The plain FP64 path on the first Bf states: the speed-ups and errors of the tiers."""
     fp64 = make_system(N, Bf, "fp64")
     fp64.set_wignerfunction(W0[:Bf] if Bf > 1 else W0[0])
     wall64, pass_ms64 = time_propagate(fp64, W, K)
-    value64 = Bf * N * N * K / wall64
-    a = mixed.wignerfunction.get()[:Bf].astype(np.complex128)
+    out = {"value": Bf * N * N * K / wall64, "unit": "gp-steps/s", "batch": Bf, "ms_per_step": wall64 / K * 1e3,
+           "avg_pass_ms": pass_ms64, "achieved_GBps": 2 * Bf * N * N * 16 / (pass_ms64 * 1e-3) / 1e9}
     b = fp64.wignerfunction.get()
     if Bf == 1:
         b = b[None]
-    err = float(np.max(np.linalg.norm((a - b).reshape(Bf, -1), axis=1)
-                       / np.linalg.norm(b.reshape(Bf, -1), axis=1)))
-    del fp64
+    errs = {name: rel_err(t["sys"], b, Bf) for name, t in tiers.items()}
+    return out, errs
 
-    # -- solve latency: config 1 (WM FP64 256^2, x,p in [-10, 10), V = x^2/2, W0 = exp(-(x-2)^2
-    #    - p^2)/pi, dt = 0.01, 1000 steps, norm/<x>/<p>/<H> recorded every step), 22 solves --
+
+def leg_latency():
+    """Config 1 solve latency (WM FP64 256^2, x,p in [-10, 10), V = x^2/2, W0 = exp(-(x-2)^2
+    - p^2)/pi, dt = 0.01, 1000 steps, norm/<x>/<p>/<H> recorded every step), 22 solves; and
+    the same grid without observables in mixed precision."""
+    import torch
     from accuratewigner_amd import WignerMoyalCUDA1D
     lat = []
     c1 = WignerMoyalCUDA1D(X_gridDIM=256, X_amplitude=10., P_gridDIM=256, P_amplitude=10., dt=0.01, t=0.,
@@ -193,7 +197,6 @@ def main():
         torch.cuda.synchronize()
         lat.append((time.perf_counter() - t0) * 1e3)
     lat = np.array(lat[2:])
-    # the same grid without per-step observables, mixed precision
     lat_nr = []
     one = make_system(256, 1, "mixed")
     st = initial_states(one, 1)
@@ -205,10 +208,19 @@ def main():
         torch.cuda.synchronize()
         lat_nr.append((time.perf_counter() - t0) * 1e3)
     lat_nr = np.array(lat_nr[2:])
+    return {"workload": "config 1 solve: WM FP64 256x256, 1000 steps, norm/<x>/<p>/<H> every step "
+                        "(20 timed solves)",
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "no_observables_mixed": {"workload": "WM mixed 256x256, 1000 steps, observables off (10 solves)",
+                                     "p50_ms": float(np.percentile(lat_nr, 50)),
+                                     "p99_ms": float(np.percentile(lat_nr, 99))}}
 
-    # -- config 2: Wigner-Bloch Gibbs state, 512^2, x, p in [-8, 8), V = x^2/2 + 0.1 x^4,
-    #    beta 0 -> 2 in d beta = 0.005 (400 steps), renormalised and <H>(beta) recorded every
-    #    step; FP64 and mixed, 5 timed solves each --
+
+def leg_gibbs():
+    """Config 2: Wigner-Bloch Gibbs state, 512^2, x, p in [-8, 8), V = x^2/2 + 0.1 x^4, beta
+    0 -> 2 in d beta = 0.005 (400 steps), renormalised and <H>(beta) recorded every step; FP64
+    and mixed, 5 timed solves each."""
+    import torch
     from accuratewigner_amd import WignerBlochCUDA1D
     gibbs, h_rec = {}, {}
     for prec in ("fp64", "mixed"):
@@ -230,10 +242,14 @@ def main():
     gibbs["mixed"]["H_rel_err_vs_fp64"] = float(np.max(np.abs(h_rec["mixed"] - h_rec["fp64"]) / np.abs(h_rec["fp64"])))
     gibbs["workload"] = ("config 2: Wigner-Bloch Gibbs state 512x512, beta 0 -> 2 in 400 steps of 0.005, "
                          "<H>(beta) and purity every step (5 timed solves)")
+    return gibbs
 
-    # -- config 3: strong-field von Neumann rho(x, x'), 1024^2, x in [-200, 200), soft-core V,
-    #    E(t) = 0.05 sin^2(pi t / T) cos(0.06 t) over 4 cycles (20944 steps of 0.02), cos^(1/8)
-    #    absorbing edge, Ehrenfest lists (dipole acceleration -<V'>) every step; FP64 and mixed --
+
+def leg_strong():
+    """Config 3: strong-field von Neumann rho(x, x'), 1024^2, x in [-200, 200), soft-core V,
+    E(t) = 0.05 sin^2(pi t / T) cos(0.06 t) over 4 cycles (20944 steps of 0.02), cos^(1/8)
+    absorbing edge, Ehrenfest lists (dipole acceleration -<V'>) every step; FP64 and mixed."""
+    import torch
     from accuratewigner_amd import RhoVNeumannCUDA1D
     from accuratewigner_amd.strong_field import rv_ground_state_rho, rv_sys_params
     p3 = rv_sys_params()
@@ -260,9 +276,13 @@ def main():
                                                 / np.max(np.abs(accel["fp64"])))
     strong["workload"] = ("config 3: von Neumann rho 1024x1024, 4-cycle pulse, %d steps of 0.02, "
                           "Ehrenfest lists every step (1 run each)" % steps3)
+    return strong
 
-    # -- config 4: adjoint gradient d<x>(T)/dE_k, WM 512^2, x, p in [-6, 6), V = x^4/4 - x^2/2,
-    #    64 field pieces ~ U(-0.1, 0.1), 2000 steps of 0.005, checkpoints every 50 steps --
+
+def leg_adjoint():
+    """Config 4: adjoint gradient d<x>(T)/dE_k, WM 512^2, x, p in [-6, 6), V = x^4/4 - x^2/2,
+    64 field pieces ~ U(-0.1, 0.1), 2000 steps of 0.005, checkpoints every 50 steps."""
+    import torch
     from accuratewigner_amd.adjoint import WignerMoyalDipoleControl
     c4 = WignerMoyalDipoleControl(X_gridDIM=512, X_amplitude=6., P_gridDIM=512, P_amplitude=6., dt=0.005,
                                   steps=2000, n_pieces=64, V0="0.25 * X * X * X * X - 0.5 * X * X",
@@ -277,57 +297,130 @@ def main():
         c4.gradient()
         torch.cuda.synchronize()
         times.append((time.perf_counter() - t0) * 1e3)
-    adjoint = {"workload": "config 4: forward + adjoint gradient, WM FP64 512x512, 2000 steps, 64 pieces, "
-                           "checkpoint 50 (3 timed runs)",
-               "p50_ms": float(np.percentile(times, 50)), "max_ms": float(np.max(times))}
-    del c4
+    return {"workload": "config 4: forward + adjoint gradient, WM FP64 512x512, 2000 steps, 64 pieces, "
+                        "checkpoint 50 (3 timed runs)",
+            "p50_ms": float(np.percentile(times, 50)), "max_ms": float(np.max(times))}
 
-    serving = serving_run(args.serve_jobs) if args.serve_jobs > 0 else None
+
+def leg_cpu(N, seconds):
+    """The oracle on the host: one core (numpy.fft) and all cores (scipy.fft workers)."""
+    info = host_info()
+    v1, n1, e1 = cpu_baseline(N, seconds)
+    # the host's share of cores (OMP_NUM_THREADS on the GPU box), at most the CPUs it has
+    ncpu = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "0")) or 16)
+    va, na, ea = cpu_baseline(N, seconds, workers=ncpu)
+    return {"value": v1, "unit": "gp-steps/s", "cores": 1, "kind": "port",
+            "sample": "oracle (NumPy complex128, numpy.fft) WM %dx%d single state, %d steps in %.1f s, "
+                      "time-independent factors precomputed" % (N, N, n1, e1),
+            "all_cores": {"value": va, "unit": "gp-steps/s", "cores": ncpu,
+                          "sample": "the same loop with scipy.fft(workers=%d), %d steps in %.1f s" % (ncpu, na, ea)},
+            **info}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--grid", type=int, default=1024)
+    ap.add_argument("--batch", type=int, default=96)
+    ap.add_argument("--fp64-batch", type=int, default=48)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=200)
+    ap.add_argument("--legs", default="all",
+                    help="comma list of the legs to run: headline, c64, fp64, latency, gibbs, strong, adjoint, "
+                         "serving, cpu (default: all); a profiling run names the legs it measures")
+    ap.add_argument("--copy-only", type=int, default=0, metavar="MIB",
+                    help="only run the copy-bandwidth kernel on MIB MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the propagators would shard over independent states, one batch slice per GPU)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    N, B, K, W = args.grid, args.batch, args.steps, args.warmup
+    if args.copy_only:
+        gbps, ms = copy_bandwidth(args.copy_only, reps=3)
+        return {"copy_MiB": args.copy_only, "copy_GBps": gbps, "ms": ms}
+    legs = set("headline,c64,fp64,latency,gibbs,strong,adjoint,serving,cpu".split(",") if args.legs == "all"
+               else args.legs.split(","))
+    if args.no_cpu:
+        legs.discard("cpu")
+    if args.serve_jobs <= 0:
+        legs.discard("serving")
+    copy_gbps, _ = copy_bandwidth()
+    done = {}
+
+    def run(name, fn, *a):
+        print("leg: %s" % name, file=sys.stderr, flush=True)
+        done[name] = fn(*a) if name in legs else None
+        return done[name]
+
+    # the complex64-state tiers: "mixed" (FP32 arithmetic) and "c64" (FP64 arithmetic)
+    tiers = {}
+    for prec, leg in (("mixed", "headline"), ("c64", "c64")):
+        if leg in legs:
+            print("leg: %s (%s)" % (leg, prec), file=sys.stderr, flush=True)
+            tiers[prec] = leg_tier(N, B, K, W, prec)
+    Bf = min(args.fp64_batch, B)
+    fp64_line, errs = None, {}
+    if "fp64" in legs:
+        print("leg: fp64", file=sys.stderr, flush=True)
+        W0 = next(iter(tiers.values()))["W0"] if tiers else initial_states(make_system(N, 1, "fp64"), B)
+        fp64_line, errs = leg_fp64(N, Bf, K, W, W0, tiers)
+    for t in tiers.values():
+        t.pop("sys")
+        t.pop("W0")
+    h = tiers.get(HEADLINE_PRECISION, {})
+    err = errs.get(HEADLINE_PRECISION)
+    run("latency", leg_latency)
+    run("gibbs", leg_gibbs)
+    run("strong", leg_strong)
+    run("adjoint", leg_adjoint)
+    run("serving", serving_run, args.serve_jobs)
 
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_wm1024_mixed_b%d.json" % B)
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+    cpu = run("cpu", leg_cpu, N, args.cpu_seconds)
 
-    cpu = None
-    if not args.no_cpu:
-        v1, n1, e1 = cpu_baseline(N, args.cpu_seconds)
-        cpu = {"value": v1, "unit": "gp-steps/s", "cores": 1, "kind": "port",
-               "sample": "oracle (NumPy complex128, numpy.fft) WM %dx%d single state, %d steps in %.1f s, "
-                         "time-independent factors precomputed" % (N, N, n1, e1),
-               "host_cpus": os.cpu_count()}
-
+    value = h.get("value")
     line = {
         "metric": METRIC, "value": value, "unit": "gp-steps/s", "n_gpus": 1, "steps": K,
-        "warmup": W, "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "warmup": W, "ms_per_step": h["wall"] / K * 1e3 if h else None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "Wigner-Moyal split-operator propagate (no Ehrenfest, normalised at end), "
                                "%dx%d grid, batch of %d states, mixed precision (complex64 state, FP64 phases)"
                                % (N, N, B),
                    "grid": [N, N], "batch": B, "precision": "mixed", "dt": 0.01,
                    "V": "0.5*X*X + 0.05*X^4", "K": "0.5*P*P"},
-        "fp64": {"value": value64, "unit": "gp-steps/s", "batch": Bf, "ms_per_step": wall64 / K * 1e3,
-                 "avg_pass_ms": pass_ms64,
-                 "achieved_GBps": 2 * Bf * N * N * 16 / (pass_ms64 * 1e-3) / 1e9},
-        "speedup_vs_fp64": value / value64,
+        "fp64": fp64_line,
+        "speedup_vs_fp64": value / fp64_line["value"] if (value and fp64_line) else None,
         "rel_l2_err_vs_fp64": err, "err_steps": W + K,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "kernel": "k_line_pass (row and column passes)", "avg_launch_ms": launch_ms,
-                     "bytes_per_launch": bytes_per_launch, "group_items": group,
-                     "launches_per_pass": launches, "avg_pass_ms": pass_ms,
-                     "bytes_per_pass": bytes_per_pass, "copy_GBps": copy_gbps,
-                     "frac_of_copy": achieved / copy_gbps},
-        "latency": {"workload": "config 1 solve: WM FP64 256x256, 1000 steps, norm/<x>/<p>/<H> every step "
-                                "(20 timed solves)",
-                    "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                    "no_observables_mixed": {"workload": "WM mixed 256x256, 1000 steps, observables off (10 solves)",
-                                             "p50_ms": float(np.percentile(lat_nr, 50)),
-                                             "p99_ms": float(np.percentile(lat_nr, 99))}},
-        "gibbs": gibbs,
-        "strong_field": strong,
-        "adjoint": adjoint,
-        "serving": serving,
+        "error_bound": 1e-6, "error_bound_met": (err <= 1e-6) if err is not None else None,
+        "tiers": {name: {"value": t["value"], "ms_per_step": t["wall"] / K * 1e3, "avg_launch_ms": t["launch_ms"],
+                         "roofline_frac": t["achieved"] / HBM_PEAK_GBPS,
+                         "speedup_vs_fp64": t["value"] / fp64_line["value"] if fp64_line else None,
+                         "rel_l2_err_vs_fp64": errs.get(name),
+                         "arithmetic": "FP32 butterflies and factors" if name == "mixed" else "FP64 butterflies and factors"}
+                  for name, t in tiers.items()},
+        "roofline": None if not h else {
+            "bound": "hbm", "achieved": h["achieved"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": h["achieved"] / HBM_PEAK_GBPS, "traffic": traffic,
+            "kernel": "k_line_pass (row and column passes)", "avg_launch_ms": h["launch_ms"],
+            "bytes_per_launch": h["bytes_per_launch"], "group_items": h["group"],
+            "launches_per_pass": h["launches"], "avg_pass_ms": h["pass_ms"],
+            "bytes_per_pass": h["bytes_per_pass"], "copy_GBps": copy_gbps,
+            "frac_of_copy": h["achieved"] / copy_gbps,
+            "step_level_frac": value * 32 / 1e9 / HBM_PEAK_GBPS},
+        "latency": done["latency"],
+        "gibbs": done["gibbs"],
+        "strong_field": done["strong"],
+        "adjoint": done["adjoint"],
+        "serving": done["serving"],
         "cpu_baseline": cpu,
     }
     return line

@@ -8,7 +8,9 @@
  * A plan is used from one stream at a time (not re-entrant per plan, safe across plans).
  *
  * Arrays are C-order [batch][rows][cols] of complex (interleaved re, im), complex128
- * (prec AW_PREC_F64) or complex64 (AW_PREC_F32).  "axis 0" FFTs run along rows index i
+ * (prec AW_PREC_F64) or complex64 (AW_PREC_F32 and AW_PREC_C64).  Factor vectors, complex
+ * tables and real masks are in the plan's arithmetic precision: FP32 for AW_PREC_F32, FP64
+ * otherwise.  "axis 0" FFTs run along rows index i
  * (strided, one FFT per column), "axis 1" FFTs along the contiguous index j, exactly the
  * two plan kinds of the reference FFT wrapper.
  */
@@ -37,6 +39,8 @@ extern "C" {
 
 #define AW_PREC_F64 0
 #define AW_PREC_F32 1
+#define AW_PREC_C64 2   /* complex64 states in HBM; twiddles, factor vectors, masks and every
+                           FFT and factor operation in FP64 (converted on load and store) */
 
 #define AW_FORWARD (-1) /* exp(-2 pi i jk/N), unnormalised (cufft.py:6) */
 #define AW_INVERSE 1    /* exp(+2 pi i jk/N), unnormalised (cufft.py:7) */
