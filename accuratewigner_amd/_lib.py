@@ -85,6 +85,7 @@ class aw_pass(ctypes.Structure):
         ("ctab", ctypes.c_void_p), ("ctab_bstride", ctypes.c_longlong), ("chirp_out", ctypes.c_double),
         ("obs_weights", ctypes.c_void_p), ("obs_nw", ctypes.c_int), ("obs_partials", ctypes.c_void_p),
         ("obs_mode", ctypes.c_int), ("obs_norm2", ctypes.c_int), ("obs_norm2_weights", ctypes.c_void_p),
+        ("in_scaled_out", ctypes.c_void_p),
     ]
 
 

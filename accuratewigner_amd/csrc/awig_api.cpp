@@ -194,6 +194,9 @@ static int pass_run_n(aw_plan *p, void *data, const aw_pass *ps, int nbatch, aw_
         a.obs_norm2 = ps->obs_norm2;
         a.obs_n2w = ps->obs_norm2_weights;
     }
+    if (ps->in_scaled_out && !ps->dscale)
+        return set_error(AW_ERR_INVALID_VALUE, "in_scaled_out needs dscale");
+    a.aux_out = ps->in_scaled_out;
     a.has_factor = (ps->line_fac || ps->elem_fac || ps->mask || ps->ctab || ps->chirp != 0.0 ||
                     ps->scale != 1.0 || ps->dscale) ? 1 : 0;
     a.tw = strided ? p->tw_rows : p->tw_cols;

@@ -57,11 +57,16 @@ int obs_lpb(int prec, int n, bool strided, long long total_lines) {
 int launch_pass(int prec, int n, bool strided, const PassArgs &a, hipStream_t st) {
     int kind = 0;
     const bool plain = a.chirp == 0.0 && !a.sign_in && !a.sign_out && a.out_mode == 0 && !a.ctab &&
-                       a.chirp_out == 0.0;
+                       a.chirp_out == 0.0 && !a.aux_out;
     if (plain) {
         if (a.dir_a == 0 && a.dir_b == AW_INVERSE) kind = 1;
         else if (a.dir_a == AW_FORWARD && a.dir_b == AW_INVERSE) kind = 2;
         else if (a.dir_a == AW_FORWARD && a.dir_b == 0) kind = 3;
+    } else if (a.out_mode == 0) {
+        // general passes with the directions of the Wigner transform's passes fixed at compile
+        // time (the kernel then holds one FFT body per slot)
+        if (a.dir_a == AW_FORWARD && a.dir_b == AW_INVERSE) kind = 7;
+        else if (a.dir_a == AW_INVERSE && a.dir_b == 0) kind = 8;
     }
     if (a.obs_w) {
         if (a.obs_mode == 0) {

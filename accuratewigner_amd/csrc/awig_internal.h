@@ -40,6 +40,7 @@ struct PassArgs {
     const double *obs_n2w;  // real weights [rows][cols] of that component, or NULL
     int obs_mode;           // 0: partials of the pass output (row passes); 1: of the values
                             // right after the factor multiply (column passes)
+    void *aux_out;          // general passes: the loaded input times dscale[b], stored here
 };
 
 struct FactorArgs {

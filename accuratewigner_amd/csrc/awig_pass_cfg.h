@@ -37,7 +37,9 @@ template <typename R, int N, bool STRIDED> struct PassTune {
     static constexpr int WANT0 = STRIDED ? WANT_COLS : WANT_ROWS;
     // at most 512 threads per block (a 1024-thread block caps a thread at 128 VGPRs)
     static constexpr int WANT = (WANT0 * T > 512) ? (512 / T > 1 ? 512 / T : 1) : WANT0;
-    static constexpr int TW_PREF = (F32 || STRIDED) ? 1 : 0;
+    // FP64 rows of 16 elements per thread exchange in halves with the twiddles in LDS (below)
+    static constexpr bool R64X = !F32 && !STRIDED && E == 16;
+    static constexpr int TW_PREF = (F32 || STRIDED || R64X) ? 1 : 0;
     static constexpr int fit(int lpb, int tw) {
         return lpb <= 1 ? 1 : (lds_bytes_dyn(lpb, tw) <= kLdsBudget ? lpb : fit(lpb / 2, tw));
     }
@@ -58,7 +60,10 @@ template <typename R, int N, bool STRIDED> struct PassTune {
     // blocks per CU): 1.00e11 -> 1.17e11 grid-point steps/s on 2048^2 x 24.  At 4096 points
     // that is 6 % slower, and FP64 columns of 2048 / 4096 points with half the columns per
     // block and the twiddles through the cache lose 15 / 32 %.
-    static constexpr int XS = (WIDE || (F32 && STRIDED && N == 2048) || (F32 && !STRIDED && E == 16)) ? 1 : 0;
+    // FP64 rows (E = 16, 2 rows per 128 threads) with the split exchange and LDS twiddles:
+    // 94.3 -> 87.6 us per 12-state 1024^2 launch (complex64 states with FP64 arithmetic:
+    // 161.8 -> 160.6 us per 24 states); either change alone gains 1-3 %
+    static constexpr int XS = (WIDE || (F32 && STRIDED && N == 2048) || (F32 && !STRIDED && E == 16) || R64X) ? 1 : 0;
     static constexpr int PF = (F32 && !STRIDED && (N == 256 || N >= 1024)) ? 1 : 0;
 };
 
@@ -122,7 +127,11 @@ int launch_cfg(const PassArgs &a, hipStream_t st) {
 //       4 row step [forward, factor, inverse] that also stores weighted-sum block partials of
 //         its output;
 //       5 / 6 column step [forward, factor, inverse] / [forward, factor] that stores the block
-//         partials of the values right after the factor multiply
+//         partials of the values right after the factor multiply;
+//       7 / 8 general pass with the directions [forward, inverse] / [inverse, none] fixed (the
+//         passes of the Wigner transform: one FFT body per slot instead of both directions,
+//         104 -> 92 us for an FP32 1024-point row pass over 24 1024^2 states); the latency
+//         configuration runs them as kind 0
 // (the partial kinds need blocks of whole waves: obs_lpb_n reports 0 where they are not)
 template <typename R, int N, bool STRIDED, typename S = R>
 int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
@@ -140,7 +149,10 @@ int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
                 }
                 return set_error(AW_ERR_INVALID_VALUE,
                                  "partials after the factor: column passes of whole waves only");
-            case 1: return launch_cfg<PassKS<R, N, STRIDED, 0, 1, 0, 0, S>>(a, st);
+            case 7:
+            case 8:
+            case 1: if (kind == 1) return launch_cfg<PassKS<R, N, STRIDED, 0, 1, 0, 0, S>>(a, st);
+                    return launch_cfg<PassKS<R, N, STRIDED, 2, 2, 1, 0, S>>(a, st);
             case 2: return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0, 0, S>>(a, st);
             case 3: return launch_cfg<PassKS<R, N, STRIDED, -1, 0, 0, 0, S>>(a, st);
             default: return launch_cfg<PassKS<R, N, STRIDED, 2, 2, 1, 0, S>>(a, st);
@@ -158,6 +170,8 @@ int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
             return launch_cfg<PassK<R, N, STRIDED, -1, 0, 0, 2, S>>(a, st);
         }
         return set_error(AW_ERR_INVALID_VALUE, "partials after the factor: column passes of whole waves only");
+    case 7: return launch_cfg<PassK<R, N, STRIDED, -1, 1, 1, 0, S>>(a, st);
+    case 8: return launch_cfg<PassK<R, N, STRIDED, 1, 0, 1, 0, S>>(a, st);
     case 1: return launch_cfg<PassK<R, N, STRIDED, 0, 1, 0, 0, S>>(a, st);
     case 2: return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0, 0, S>>(a, st);
     case 3: return launch_cfg<PassK<R, N, STRIDED, -1, 0, 0, 0, S>>(a, st);

@@ -435,6 +435,16 @@ void k_line_pass(PassArgs a) {
     T2 v[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) v[k] = cvt2<T2>(src[k * kstep]);
+    if (C::GEN && a.aux_out && active) {
+        // the input times dscale[b], in FP64 and rounded once (as a separate scale pass would)
+        const double2 ds = reinterpret_cast<const double2 *>(a.dscale)[b];
+        S2 *aux = reinterpret_cast<S2 *>(a.aux_out) + item + off0;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const double x = v[k].x, y = v[k].y;
+            aux[k * kstep] = mk<S2>(x * ds.x - y * ds.y, x * ds.y + y * ds.x);
+        }
+    }
     // optional prefetch of the element factors: their load latency then overlaps the data
     // loads instead of stalling between the two FFTs (costs 2E registers)
     T2 efp[C::PF ? E : 1];

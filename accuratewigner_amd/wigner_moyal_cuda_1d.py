@@ -134,6 +134,7 @@ class WignerMoyalCUDA1D(KeywordParams):
         self._observable_kernels = {}
         self._obs_partials = {}
         self._g1 = None           # L^H 1: the norm weights of normalize_rho_wigner
+        self._group_plans = {}    # plans over one MALL-resident group of items
         self._ehr = None          # persistent chunk buffers and captured graphs of the Ehrenfest loop
         self._prop = None         # the same for the merged-step loop
         self._prepare_masks()
@@ -470,9 +471,9 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
     def normalize_rho_wigner(self):
         """wigner_moyal_cuda_1d.py:608-622: W = rho2wigner; n = Re sum W dXdP; W /= n;
         rho /= n (device-side, no host sync).  n is read from rho as Re <L^H 1, rho> dXdP (the
-        transform L is linear), one launch pair; W is then formed already divided by n (the
-        scale rides the transform's first pass) and rho is divided in one pass, instead of a
-        read of W and two scale passes after the transform."""
+        transform L is linear), one launch pair; W is then formed already divided by n, and the
+        transform's first pass, which reads rho, also writes it back divided by n: no read of W
+        and no scale passes after the transform."""
         if not self.obs_by_adjoint:
             self.rho2wigner()
             st = self.plan.stats(self.wignerfunction)
@@ -483,8 +484,8 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         self._flush_scale()
         g1 = self._norm_weights()
         self.plan.weighted_sums(self.rho, g1, 1, self._norm_out, self.dXdP, self._dscale)
-        wm_ops.rho2wigner(self.plan, self._grid, self.rho, self.wignerfunction, dscale=self._dscale)
-        self.plan.scale(self.rho, self._dscale)
+        wm_ops.rho2wigner(self.plan, self._grid, self.rho, self.wignerfunction, dscale=self._dscale,
+                          scale_rho=True)
         return self
 
     # ------------------------------------------------------------------------------------
@@ -636,21 +637,28 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
                 self._prop = dict(self._chunk_bufs(CH), graphs={})
             # latency-bound sizes (the small-pass kernels) replay a captured chunk
             small = self.batch * max(self.X_gridDIM, self.P_gridDIM) <= 2048
+            normalized = False
             for c0 in range(0, steps, CH):
                 n = min(CH, steps - c0)
                 self._build_chunk_factors(ts_all.tensor[c0:c0 + n], merged=True, bufs=self._prop)
-                ev = self._pass_events
-                if ev is not None:
-                    import torch
-                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                    e0.record()
-                graphs.run_chunk(self._prop["graphs"], "merged", lambda n=n: self._merged_chunk(n),
-                                 n == CH and small)
-                if ev is not None:
-                    e1.record()
-                    ev.append((e0, e1, 2 * n + 1))
+                groups = self._item_groups(n) if (c0 + n == steps and self.obs_by_adjoint) else None
+                if groups:
+                    # the last chunk of a batch that outgrows the MALL: each group of items runs
+                    # its steps, then at once its normalisation and Wigner transform, while it
+                    # is still in the MALL
+                    self._flush_scale()
+                    for b0, nb in groups:
+                        self._timed_passes(lambda b0=b0, nb=nb: self._merged_chunk(n, b0, nb),
+                                           (2 * n + 1) * nb / self.batch)
+                        self._normalize_items(b0, nb)
+                    normalized = True
+                    continue
+                self._timed_passes(lambda n=n: graphs.run_chunk(self._prop["graphs"], "merged",
+                                                                lambda: self._merged_chunk(n), n == CH and small),
+                                   2 * n + 1)
             self.t = ts[-1]
-            self.normalize_rho_wigner()
+            if not normalized:
+                self.normalize_rho_wigner()
             return self.wignerfunction
         # Ehrenfest: per step normalise through the Wigner transform and record
         # <X>, <diff_K>, <P>, -<diff_V>, <K + V> (:584-606) into a device ring; one host copy
@@ -705,14 +713,52 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             lst.extend(vals[:, 0].tolist() if self.batch == 1 else list(vals))
         return self.wignerfunction
 
-    def _merged_chunk(self, n):
-        """n merged steps (2 n + 1 passes) from the chunk buffers."""
+    def _merged_chunk(self, n, b0=0, nb=0):
+        """n merged steps (2 n + 1 passes) from the chunk buffers, over the items b0 .. b0 + nb
+        (nb = 0: all)."""
         colL, colE, rowL, rowE = self._prop["fac"]
         _, _, tab1 = self._mask_args(0, 1)
         _, _, tab2 = self._mask_args(0, 2)
         _, _, tabp = self._mask_args(1, 1)
-        self.plan.steps(self.rho, n, colL, colE, rowL, rowE, mask_open_close=tab1, mask_mid=tab2,
-                        mask_row=tabp)
+        self.plan.steps(self.rho.ptr + b0 * self._item_bytes(), n, colL, colE, rowL, rowE, mask_open_close=tab1,
+                        mask_mid=tab2, mask_row=tabp, batch=nb)
+
+    def _item_bytes(self):
+        return self.X_gridDIM * self.P_gridDIM * np.dtype(self.state_dtype).itemsize
+
+    def _timed_passes(self, launch, npass):
+        """Run ``launch``; when a caller collects pass timings (_pass_events), bracket it with
+        events and count it as npass passes over the whole batch."""
+        ev = self._pass_events
+        if ev is None:
+            launch()
+            return
+        import torch
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        launch()
+        e1.record()
+        ev.append((e0, e1, npass))
+
+    def _item_groups(self, n):
+        """[(first item, items)] of the MALL-resident groups in which the step sequence runs
+        a batch for n steps, or None when it runs the batch whole."""
+        g = self.plan.step_group(n, self.batch)
+        if g >= self.batch:
+            return None
+        return [(b0, min(g, self.batch - b0)) for b0 in range(0, self.batch, g)]
+
+    def _normalize_items(self, b0, nb):
+        """normalize_rho_wigner on the items b0 .. b0 + nb: the norm from rho, W formed already
+        divided by it, and rho divided (a plan of nb items over the group's slices)."""
+        gp = self._group_plans.get(nb)
+        if gp is None:
+            gp = self._group_plans[nb] = Plan(self.P_gridDIM, self.X_gridDIM, nb, self.precision)
+        off = b0 * self._item_bytes()
+        rho, W = self.rho.ptr + off, self.wignerfunction.ptr + off
+        ds = self._dscale.ptr + 16 * b0
+        gp.weighted_sums(rho, self._norm_weights(), 1, self._norm_out.ptr + 8 * b0, self.dXdP, ds)
+        wm_ops.rho2wigner(gp, self._grid, rho, W, dscale=ds, scale_rho=True)
 
     def _ehrenfest_chunk(self, exprs, n):
         """n steps, each [EV, I0] [F1, EK, I1] [F0, EV] from the chunk buffers (applying the

@@ -69,11 +69,11 @@ def wigner2rho(plan, g, W, rho):
     shear_x(plan, g, rho, INV, FWD, sign_out=1)
 
 
-def rho2wigner(plan, g, rho, W, dscale=None):
+def rho2wigner(plan, g, rho, W, dscale=None, scale_rho=False):
     """wigner_moyal_cuda_1d.py:471-501: [S, F1, SX, I1] [F0, SY, I0] [F1, SX, I1] [I0, S]
     (unnormalised, like the reference; ``dscale``: complex per-item factors applied in the
-    first pass)."""
-    shear_x(plan, g, rho, FWD, INV, sign_in=1, out=W, dscale=dscale)
+    first pass, which with ``scale_rho`` also writes rho times them back into rho)."""
+    shear_x(plan, g, rho, FWD, INV, sign_in=1, out=W, dscale=dscale, in_scaled_out=rho if scale_rho else None)
     shear_y(plan, g, W, FWD, INV)
     shear_x(plan, g, W, FWD, INV)
     plan.run(W, 0, INV, 0, sign_out=1)

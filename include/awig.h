@@ -116,6 +116,11 @@ typedef struct aw_pass {
     double *obs_partials;
     int obs_mode, obs_norm2;
     const double *obs_norm2_weights;
+    /* With dscale: also store each loaded element times dscale[b] (before the sign flip and
+     * the first FFT) into in_scaled_out (the input's layout; may be the input itself), e.g. the
+     * division of rho by its norm riding on the first pass of its Wigner transform
+     * (normalize_rho_wigner, wigner_moyal_cuda_1d.py:608-622).  NULL: none. */
+    void *in_scaled_out;
 } aw_pass;
 
 int aw_pass_run(aw_plan *plan, void *data, const aw_pass *pass, aw_stream stream);

@@ -194,3 +194,28 @@ def test_pass_partials_refused_where_blocks_are_not_whole_waves():
                dict(obs_nw=1, obs_norm2_weights=w2)):        # weights without the component
         with pytest.raises(_lib.awigInvalidValue):
             p.run(z, 1, FWD, INV, obs_weights=w, obs_partials=part, **kw)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "c64"])
+def test_pass_in_scaled_out_equals_scale_pass(prec):
+    """A general pass that also stores its input times dscale (the division of rho by its
+    norm on the first pass of the Wigner transform) leaves the same bits as a scale pass, and
+    its own output equals the pass without the extra store."""
+    from accuratewigner_amd.device import to_device
+    from accuratewigner_amd.engine import FWD, INV, Plan
+    rows, cols, B = 128, 256, 3
+    rng = np.random.default_rng(8)
+    p = Plan(rows, cols, B, prec)
+    z = _rand(rng, (B, rows, cols)).astype(p.state_dtype)
+    ds = to_device(np.stack([rng.uniform(0.5, 2.0, B), rng.uniform(-0.5, 0.5, B)], axis=1))
+    kw = dict(chirp=0.3, lstep=0.01, lcenter=64., estep=0.02, ecenter=128., sign_in=1, dscale=ds)
+    rho, W = to_device(z), to_device(np.zeros_like(z))
+    p.run(rho, 1, FWD, INV, out=W, in_scaled_out=rho, **kw)
+    rho2, W2 = to_device(z), to_device(np.zeros_like(z))
+    p.run(rho2, 1, FWD, INV, out=W2, **kw)
+    p.scale(rho2, ds)
+    # both round the FP64 product once (the compiler may contract either into an FMA)
+    a, b = rho.get().astype(np.complex128), rho2.get().astype(np.complex128)
+    assert np.max(np.abs(a - b)) <= 1e-15 * np.max(np.abs(b))
+    assert np.array_equal(W.get(), W2.get())
+    assert not np.array_equal(rho.get(), z)
