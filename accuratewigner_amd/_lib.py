@@ -126,6 +126,7 @@ _SIGNATURES = {
     "aw_cyclic_diag_sums": [_vp, _vp, _vp, _vp],
     "aw_weighted_sums": [_vp, _vp, _vp, _i, _i, _vp, _d, _vp, _vp],
     "aw_weighted_partials": [_vp, _vp, _vp, _i, _i, _vp, ctypes.POINTER(_i), _vp],
+    "aw_weighted_sums_noalloc": [_i, _i, _i, _i, _vp, _vp, _i, _i, _vp, _d, _vp, _vp],
     "aw_cyclic_diag_obs": [_vp, _vp, _vp, _i, _vp, _i, _vp, _ll, _d, _vp, _vp],
     "aw_scale_from_stats": [_vp, _i, _i, _i, _d, _vp, _i, _vp],
     "aw_scale": [_i, _vp, _ll, _i, _vp, _d, _vp],

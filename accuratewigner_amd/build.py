@@ -5,7 +5,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SOURCES = [os.path.join(HERE, "csrc", f) for f in
-           ("awig_fft.hip", "awig_pass_f32.hip", "awig_pass_f64.hip", "awig_pass_c64.hip", "awig_api.cpp")]
+           ("awig_fft.hip", "awig_pass_f32.hip", "awig_pass_f64.hip", "awig_pass_c64.hip", "awig_scratch.hip",
+                      "awig_api.cpp")]
 TARGET = os.path.join(HERE, "libawig.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -229,6 +229,19 @@ int aw_weighted_sums(aw_plan *plan, const void *data, const double *weights, int
                      double factor, double *dscale, aw_stream stream);
 
 /*
+ * aw_weighted_sums without a plan and without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The block partials live in
+ * a fixed device array of 512 x 8 x 256 doubles (8 MiB, held by each GPU the library runs on
+ * until the process ends), so nblk * (nw + norm2) * batch must fit in it (nblk = rows * cols /
+ * 2048, clamped to 1..512), else AW_ERR_INVALID_SIZE.  The results are bit-identical to
+ * aw_weighted_sums on a plan of the same shape and precision.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (different streams, or threads) race.
+ */
+int aw_weighted_sums_noalloc(int prec, int rows, int cols, int batch, const void *data, const double *weights,
+                             int nw, int norm2, double *out, double factor, double *dscale, aw_stream stream);
+
+/*
  * The first stage of aw_weighted_sums alone: block partials
  * partials[b][k][c] (c < nw + norm2, k < *nblk) into a caller buffer, one launch, no scratch
  * (so any call may be captured).  *nblk receives the block count (partials = NULL: only that).
