@@ -295,10 +295,18 @@ struct KCfg {
 // belong to one item):  Re sum conj(w_q) v for q < obs_nw, then (obs_norm2) sum |v|^2, with
 // real weights obs_n2w when given, into obs_partials[item][block][component].  Each thread
 // adds its elements, then the waves and then the waves of the block are added in order.
+// Row step passes of up to 4 elements per thread (the single-state latency configuration)
+// load the first kObsPre weight arrays with the data, so that their latency overlaps the FFTs
+// instead of following the second one.
+constexpr int kObsPre = 4;
+template <typename C> constexpr int obs_prefetch() { return (C::OBS == 1 && C::E <= 4) ? kObsPre : 0; }
+
 template <typename C, typename T2>
 __device__ __forceinline__ void obs_block_partials(const PassArgs &a, const T2 *v, bool active, int off0,
-                                                   int kstep, long long blk, int nl, int tid) {
+                                                   int kstep, long long blk, int nl, int tid,
+                                                   const double2 (*wpre)[C::E] = nullptr) {
     constexpr int N = C::N, E = C::E, LPB = C::LPB;
+    constexpr int NPRE = obs_prefetch<C>();
     static_assert(C::THREADS % 64 == 0, "weighted-sum partials of whole waves");
     const int nw = a.obs_nw, nc = a.obs_nw + a.obs_norm2;
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -308,7 +316,13 @@ __device__ __forceinline__ void obs_block_partials(const PassArgs &a, const T2 *
 #pragma unroll
         for (int k = 0; k < E; ++k) {
             const double x = v[k].x, y = v[k].y;
-            for (int q = 0; q < nw; ++q) {
+            if constexpr (NPRE > 0) {
+#pragma unroll
+                for (int q = 0; q < NPRE; ++q) {
+                    if (q < nw) acc[q] += wpre[q][k].x * x + wpre[q][k].y * y;
+                }
+            }
+            for (int q = NPRE; q < nw; ++q) {
                 const double2 c = w[q * nit + (long long)k * kstep];
                 acc[q] += c.x * x + c.y * y;
             }
@@ -435,6 +449,20 @@ void k_line_pass(PassArgs a) {
     T2 v[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) v[k] = cvt2<T2>(src[k * kstep]);
+    constexpr int NPRE = obs_prefetch<C>();
+    double2 wpre[NPRE > 0 ? NPRE : 1][E];
+    if constexpr (NPRE > 0) {
+        if (active) {
+            const long long nit = (long long)N * nl;
+#pragma unroll
+            for (int q = 0; q < NPRE; ++q) {
+                if (q < a.obs_nw) {
+#pragma unroll
+                    for (int k = 0; k < E; ++k) wpre[q][k] = a.obs_w[q * nit + off0 + (long long)k * kstep];
+                }
+            }
+        }
+    }
     if (C::GEN && a.aux_out && active) {
         // the input times dscale[b], in FP64 and rounded once (as a separate scale pass would)
         const double2 ds = reinterpret_cast<const double2 *>(a.dscale)[b];
@@ -546,7 +574,7 @@ void k_line_pass(PassArgs a) {
             if (par & 1) v[k] = cscale(v[k], R(-1));
         }
     }
-    if constexpr (C::OBS == 1) obs_block_partials<C>(a, v, active, off0, kstep, blk, nl, tid);
+    if constexpr (C::OBS == 1) obs_block_partials<C>(a, v, active, off0, kstep, blk, nl, tid, wpre);
     if (!active) return;
     if (!C::GEN || a.out_mode == 0) {
         S2 *dst = reinterpret_cast<S2 *>(a.out) + item + off0;

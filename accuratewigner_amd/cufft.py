@@ -1,10 +1,18 @@
 """Drop-in for the reference's cuFFT wrapper (cufft.py) on libawig's FFT engine.
 
-Same public names: CUFFT_FORWARD / CUFFT_INVERSE, the cufftError hierarchy, cufftCheckStatus,
-Plan_Z2Z_2D_Axis0 / Plan_Z2Z_2D_Axis1 (cufft.py:274-345) and fft_Z2Z / ifft_Z2Z
-(cufft.py:233-238): batched, unnormalised complex128 FFTs along one axis of a 2D device
-array, forward = exp(-2 pi i jk/N).  Arrays are ``DeviceArray`` (or any object with
-``.gpudata`` and ``.shape``); complex64 arrays use the FP32 engine.
+Same public names: the CUFFT_* constants (cufft.py:6-14), the cufftError hierarchy and
+cufftCheckStatus (:31-104), cufftPlanMany / cufftDestroy / cufftExecZ2Z (:107-127, :224-232),
+PlanZ2Z (:243-271), Plan_Z2Z_2D_Axis0 / Plan_Z2Z_2D_Axis1 (:274-345) and fft_Z2Z / ifft_Z2Z
+(:233-238): batched, unnormalised complex FFTs, forward = exp(-2 pi i jk/N).  Arrays are
+``DeviceArray`` (or any object with ``.gpudata`` and ``.shape``); CUFFT_C2C plans (complex64)
+use the FP32 engine.
+
+cufftPlanMany accepts the layouts the reference builds: rank 1 over contiguous batches
+(istride 1, idist n), rank 1 over the strided columns of a 2D array (idist 1, istride = the
+number of columns = batch), and contiguous rank-2 batches.  Lengths are powers of two from 64
+to 4096 (the column count of a strided plan as well); other layouts raise cufftInvalidSize,
+other types cufftInvalidType.  n, inembed and onembed may be host addresses of int32 arrays
+(as the reference passes them) or sequences.
 """
 import ctypes
 
@@ -15,8 +23,13 @@ from .device import current_stream
 
 CUFFT_FORWARD = -1
 CUFFT_INVERSE = 1
-CUFFT_C2C = 0x29
-CUFFT_Z2Z = 0x69
+
+CUFFT_R2C = 0x2a  # real to complex (interleaved)
+CUFFT_C2R = 0x2c  # complex (interleaved) to real
+CUFFT_C2C = 0x29  # complex to complex (interleaved)
+CUFFT_D2Z = 0x6a  # double to double-complex (interleaved)
+CUFFT_Z2D = 0x6c  # double-complex (interleaved) to double
+CUFFT_Z2Z = 0x69  # double-complex to double-complex (interleaved)
 
 
 class cufftError(_lib.awigError):
@@ -128,3 +141,111 @@ def fft_Z2Z(x_input_gpu, y_output_gpu, plan):
 def ifft_Z2Z(x_input_gpu, y_output_gpu, plan):
     """Inverse unnormalised FFT (cufft.py:237-238)."""
     _exec(x_input_gpu, y_output_gpu, plan, CUFFT_INVERSE)
+
+
+# ---------------------------------------------------------------------------------------
+# the plan-handle interface (cufftPlanMany / cufftExecZ2Z / cufftDestroy)
+# ---------------------------------------------------------------------------------------
+class _ManyPlan:
+    """What a cufftPlanMany handle stands for: an aw_plan and the passes one execution runs."""
+
+    def __init__(self, rows, cols, batch, prec, axes):
+        h = ctypes.c_void_p()
+        cufftCheckStatus(_lib.load().aw_plan_create(ctypes.byref(h), rows, cols, batch, prec))
+        self.handle, self.axes = h, axes
+
+    def __del__(self):
+        try:
+            _lib.load().aw_plan_destroy(self.handle)
+        except Exception:
+            pass
+
+
+_PLANS = {}
+_NEXT_HANDLE = [1]
+
+
+def _ints(v, count):
+    """count ints from a host address of int32 values (how the reference passes n, inembed and
+    onembed, e.g. ``n.ctypes.data``) or from a sequence; None stays None."""
+    if v is None:
+        return None
+    if isinstance(v, (int, np.integer)):
+        return list((ctypes.c_int32 * count).from_address(int(v)))
+    return [int(x) for x in list(v)[:count]]
+
+
+def cufftPlanMany(rank, n, inembed, istride, idist, onembed, ostride, odist, fft_type, batch):
+    """Create a batched FFT plan (cufft.py:107-116); returns the handle as a ctypes.c_uint."""
+    if fft_type not in (CUFFT_Z2Z, CUFFT_C2C):
+        raise cufftInvalidType("only complex-to-complex plans (CUFFT_Z2Z, CUFFT_C2C) are supported")
+    prec = _lib.AW_PREC_F64 if fft_type == CUFFT_Z2Z else _lib.AW_PREC_F32
+    dims = _ints(n, rank)
+    if rank not in (1, 2) or batch < 1 or (istride, idist) != (ostride, odist):
+        raise cufftInvalidSize("unsupported plan: rank %d, batch %d" % (rank, batch))
+    ie = _ints(inembed, rank)
+    oe = _ints(onembed, rank)
+    if rank == 2 and ((ie and ie[1:] != dims[1:]) or (oe and oe[1:] != dims[1:])):
+        raise cufftInvalidSize("rank-2 plans must be contiguous")
+    if rank == 1 and istride == 1 and idist in (0, dims[0]):
+        plan = _ManyPlan(1, dims[0], batch, prec, (1,))                  # rows of contiguous batches
+    elif rank == 1 and idist == 1 and istride == batch:
+        plan = _ManyPlan(dims[0], batch, 1, prec, (0,))                  # the columns of [n, batch]
+    elif rank == 2 and istride == 1 and idist in (0, dims[0] * dims[1]):
+        plan = _ManyPlan(dims[0], dims[1], batch, prec, (1, 0))          # contiguous 2D transforms
+    else:
+        raise cufftInvalidSize("unsupported layout: istride %d, idist %d" % (istride, idist))
+    h = _NEXT_HANDLE[0]
+    _NEXT_HANDLE[0] += 1
+    _PLANS[h] = plan
+    return ctypes.c_uint(h)
+
+
+def _plan_of(plan):
+    h = plan.value if isinstance(plan, ctypes.c_uint) else int(plan)
+    p = _PLANS.get(h)
+    if p is None:
+        raise cufftInvalidPlan("invalid plan handle %d" % h)
+    return h, p
+
+
+def cufftDestroy(plan):
+    """Destroy an FFT plan (cufft.py:119-127)."""
+    h, _ = _plan_of(plan)
+    del _PLANS[h]
+
+
+def cufftExecZ2Z(plan, idata, odata, direction):
+    """Execute a complex-to-complex plan on device addresses (cufft.py:224-232): in place when
+    idata == odata.  A rank-2 plan transforms along the rows, then along the columns."""
+    _, p = _plan_of(plan)
+    if direction not in (CUFFT_FORWARD, CUFFT_INVERSE):
+        raise cufftInvalidValue("direction must be CUFFT_FORWARD or CUFFT_INVERSE")
+    lib = _lib.load()
+    src, dst = int(idata), int(odata)
+    for k, axis in enumerate(p.axes):
+        a = src if k == 0 else dst
+        out = dst if dst != a else None
+        cufftCheckStatus(lib.aw_fft_axis(p.handle, ctypes.c_void_p(a), ctypes.c_void_p(out), axis, direction,
+                                         ctypes.c_void_p(current_stream())))
+
+
+class PlanZ2Z:
+    """Batched complex128 plan of rank len(shape) (cufft.py:243-271)."""
+
+    def __init__(self, shape, batch=1):
+        self.shape = shape
+        self.batch = batch
+        self.fft_type = CUFFT_Z2Z
+        if len(self.shape) > 0:
+            n = np.asarray(shape, np.int32)
+            self.handle = cufftPlanMany(len(shape), n.ctypes.data, None, 1, 0, None, 1, 0, self.fft_type,
+                                        self.batch)
+        else:
+            raise ValueError('invalid transform size')
+
+    def __del__(self):
+        try:
+            cufftDestroy(self.handle)
+        except Exception:
+            pass

@@ -662,7 +662,11 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             return self.wignerfunction
         # Ehrenfest: per step normalise through the Wigner transform and record
         # <X>, <diff_K>, <P>, -<diff_V>, <K + V> (:584-606) into a device ring; one host copy
-        exprs = ["1.", "X", self.diff_K, "P", self.diff_V, self.hamiltonian]
+        wanted = ["1.", "X", self.diff_K, "P", self.diff_V, self.hamiltonian]
+        # each distinct expression is summed once (config 1 has diff_K = "P" and diff_V = "X":
+        # 4 weight arrays instead of 6)
+        exprs = list(dict.fromkeys(e.strip() for e in wanted))
+        col = [exprs.index(e.strip()) for e in wanted]
         nobs = len(exprs)
         ring = empty((steps, self.batch, nobs), np.float64)
         CH = CHUNK_STEPS
@@ -709,7 +713,7 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         for lst, k, sgn in ((self.X_average, 1, 1.), (self.X_average_RHS, 2, 1.),
                             (self.P_average, 3, 1.), (self.P_average_RHS, 4, -1.),
                             (self.hamiltonian_average, 5, 1.)):
-            vals = sgn * r[:, :, k] / s0
+            vals = sgn * r[:, :, col[k]] / s0
             lst.extend(vals[:, 0].tolist() if self.batch == 1 else list(vals))
         return self.wignerfunction
 

@@ -116,3 +116,11 @@ def test_entry_points_reject_null_and_bad_arguments_before_touching_the_gpu():
         with pytest.raises(exc):
             _lib.check(call())
         assert _lib.load().aw_last_error()          # a message is left for the caller
+
+
+def test_cufft_plan_many_rejects_real_types_without_a_gpu_call():
+    from accuratewigner_amd import cufft
+    import numpy as np
+    for t in (cufft.CUFFT_R2C, cufft.CUFFT_C2R, cufft.CUFFT_D2Z, cufft.CUFFT_Z2D):
+        with __import__("pytest").raises(cufft.cufftInvalidType):
+            cufft.cufftPlanMany(1, np.array([64], np.int32).ctypes.data, None, 1, 0, None, 1, 0, t, 1)
