@@ -245,15 +245,18 @@ def leg_gibbs():
     return gibbs
 
 
-def leg_strong():
+def leg_strong(max_steps=0):
     """Config 3: strong-field von Neumann rho(x, x'), 1024^2, x in [-200, 200), soft-core V,
     E(t) = 0.05 sin^2(pi t / T) cos(0.06 t) over 4 cycles (20944 steps of 0.02), cos^(1/8)
-    absorbing edge, Ehrenfest lists (dipole acceleration -<V'>) every step; FP64 and mixed."""
+    absorbing edge, Ehrenfest lists (dipole acceleration -<V'>) every step; FP64 and mixed.
+    max_steps > 0 cuts the pulse (profiling runs only)."""
     import torch
     from accuratewigner_amd import RhoVNeumannCUDA1D
     from accuratewigner_amd.strong_field import rv_ground_state_rho, rv_sys_params
     p3 = rv_sys_params()
     steps3 = int(round(p3["T_pulse"] / p3["dt"]))
+    if max_steps > 0:
+        steps3 = min(steps3, max_steps)
     rho0 = rv_ground_state_rho(p3, nsteps=5000)
     strong, accel = {}, {}
     for prec in ("fp64", "mixed"):
@@ -328,6 +331,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=200)
+    ap.add_argument("--strong-steps", type=int, default=0,
+                    help="cut the von Neumann leg to this many steps (profiling runs; 0: the whole pulse)")
     ap.add_argument("--legs", default="all",
                     help="comma list of the legs to run: headline, c64, fp64, latency, gibbs, strong, adjoint, "
                          "serving, cpu (default: all); a profiling run names the legs it measures")
@@ -377,7 +382,7 @@ def main():
     err = errs.get(HEADLINE_PRECISION)
     run("latency", leg_latency)
     run("gibbs", leg_gibbs)
-    run("strong", leg_strong)
+    run("strong", leg_strong, args.strong_steps)
     run("adjoint", leg_adjoint)
     run("serving", serving_run, args.serve_jobs)
 
