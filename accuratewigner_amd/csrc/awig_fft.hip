@@ -19,6 +19,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -732,7 +733,135 @@ __global__ __launch_bounds__(RED_THREADS) void k_project_out(void *psi_, const v
     block_normalize(psi, n);
 }
 
+// Multi-block form for long lines (n >= kProjMultiMin): G blocks each take a slice of psi.
+//  k_proj_partials: per block, sum |psi|^2 and sum conj(phi_q) psi over its slice
+//  k_proj_update:   every block adds the G partials in the same fixed order (so all blocks
+//                   hold the same norm and coefficients c_q = <phi_q|psi> / ||psi||), writes
+//                   psi / ||psi|| - sum c_q phi_q over its slice and its partial of the new norm
+//  k_proj_scale:    every block adds those partials and divides its slice by the norm
+// (three launches of G blocks instead of one workgroup walking the whole line for every q).
+constexpr int kProjMultiMin = 8192;
+constexpr int kProjBlocks = 128;
+
+template <typename R>
+__global__ __launch_bounds__(RED_THREADS) void k_proj_partials(const void *psi_, const void *phis_, int nphi, int n,
+                                                                double *part) {
+    using T2 = typename Cx<R>::T;
+    const T2 *psi = reinterpret_cast<const T2 *>(psi_);
+    const T2 *phis = reinterpret_cast<const T2 *>(phis_);
+    const int per = (n + gridDim.x - 1) / gridDim.x;
+    const int k0 = blockIdx.x * per, k1 = min(n, k0 + per);
+    double acc = 0.0;
+    for (int k = k0 + threadIdx.x; k < k1; k += RED_THREADS)
+        acc += double(psi[k].x) * psi[k].x + double(psi[k].y) * psi[k].y;
+    double2 r = block_sum2(acc, 0.0);
+    double *pb = part + (long long)blockIdx.x * 2 * (nphi + 1);
+    if (threadIdx.x == 0) { pb[0] = r.x; pb[1] = 0.0; }
+    for (int q = 0; q < nphi; ++q) {
+        const T2 *phi = phis + (long long)q * n;
+        double cr = 0.0, ci = 0.0;
+        for (int k = k0 + threadIdx.x; k < k1; k += RED_THREADS) {
+            const double ax = phi[k].x, ay = phi[k].y, bx = psi[k].x, by = psi[k].y;
+            cr += ax * bx + ay * by;
+            ci += ax * by - ay * bx;
+        }
+        r = block_sum2(cr, ci);
+        if (threadIdx.x == 0) { pb[2 * (q + 1)] = r.x; pb[2 * (q + 1) + 1] = r.y; }
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(RED_THREADS) void k_proj_update(void *psi_, const void *phis_, int nphi, int n,
+                                                              const double *part, double *part2) {
+    using T2 = typename Cx<R>::T;
+    extern __shared__ double coef[];   // [0]: 1 / ||psi||, then (re, im) of c_q
+    T2 *psi = reinterpret_cast<T2 *>(psi_);
+    const T2 *phis = reinterpret_cast<const T2 *>(phis_);
+    const int G = gridDim.x, stride = 2 * (nphi + 1);
+    // every block adds the partials in block order (the same sums in every block)
+    for (int c = threadIdx.x; c < stride; c += RED_THREADS) {
+        double s = 0.0;
+        for (int g = 0; g < G; ++g) s += part[(long long)g * stride + c];
+        coef[c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) coef[0] = 1.0 / sqrt(coef[0]);
+    __syncthreads();
+    const double inv = coef[0];
+    const int per = (n + G - 1) / G;
+    const int k0 = blockIdx.x * per, k1 = min(n, k0 + per);
+    double acc = 0.0;
+    for (int k = k0 + threadIdx.x; k < k1; k += RED_THREADS) {
+        double x = psi[k].x * inv, y = psi[k].y * inv;
+        for (int q = 0; q < nphi; ++q) {
+            const T2 f = phis[(long long)q * n + k];
+            const double c_r = coef[2 * (q + 1)] * inv, c_i = coef[2 * (q + 1) + 1] * inv;
+            x -= c_r * f.x - c_i * f.y;
+            y -= c_r * f.y + c_i * f.x;
+        }
+        const T2 o = mk<T2>(x, y);
+        psi[k] = o;
+        acc += double(o.x) * o.x + double(o.y) * o.y;
+    }
+    const double2 r = block_sum2(acc, 0.0);
+    if (threadIdx.x == 0) part2[blockIdx.x] = r.x;
+}
+
+template <typename R>
+__global__ __launch_bounds__(RED_THREADS) void k_proj_scale(void *psi_, int n, const double *part2) {
+    using T2 = typename Cx<R>::T;
+    __shared__ double inv;
+    T2 *psi = reinterpret_cast<T2 *>(psi_);
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int g = 0; g < (int)gridDim.x; ++g) s += part2[g];
+        inv = 1.0 / sqrt(s);
+    }
+    __syncthreads();
+    const int per = (n + gridDim.x - 1) / gridDim.x;
+    const int k0 = blockIdx.x * per, k1 = min(n, k0 + per);
+    for (int k = k0 + threadIdx.x; k < k1; k += RED_THREADS) psi[k] = mk<T2>(psi[k].x * inv, psi[k].y * inv);
+}
+
+// device scratch of the multi-block projection: kProjBlocks x (nphi + 1) complex partials and
+// kProjBlocks norm partials, allocated on first use per device for the largest nphi (4096)
+static double *g_proj_part[64];
+static std::mutex g_proj_mutex;
+
+template <typename R>
+static int launch_project_multi(void *psi, const void *phis, int nphi, int n, hipStream_t st) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= 64) return set_error(AW_ERR_HIP, "hipGetDevice failed");
+    {
+        std::lock_guard<std::mutex> lock(g_proj_mutex);
+        if (!g_proj_part[dev]) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            e = hipStreamIsCapturing(st, &cs);
+            if (e != hipSuccess || cs != hipStreamCaptureStatusNone)
+                return set_error(AW_ERR_INVALID_VALUE, "the first multi-block aw_project_out call must not be captured");
+            e = hipMalloc((void **)&g_proj_part[dev], sizeof(double) * (2ll * kProjBlocks * 4097 + kProjBlocks));
+            if (e != hipSuccess) {
+                g_proj_part[dev] = nullptr;
+                return set_error(AW_ERR_ALLOC_FAILED, "projection scratch: %s", hipGetErrorString(e));
+            }
+        }
+    }
+    double *part = g_proj_part[dev], *part2 = part + 2ll * kProjBlocks * 4097;
+    const int G = kProjBlocks;
+    hipLaunchKernelGGL(k_proj_partials<R>, dim3(G), dim3(RED_THREADS), 0, st, psi, phis, nphi, n, part);
+    hipLaunchKernelGGL(k_proj_update<R>, dim3(G), dim3(RED_THREADS), sizeof(double) * 2 * (nphi + 1), st, psi,
+                       phis, nphi, n, part, part2);
+    hipLaunchKernelGGL(k_proj_scale<R>, dim3(G), dim3(RED_THREADS), 0, st, psi, n, part2);
+    e = hipGetLastError();
+    if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "multi-block projection: %s", hipGetErrorString(e));
+    return AW_OK;
+}
+
 int launch_project_out(int prec, void *psi, const void *phis, int nphi, int n, hipStream_t st) {
+    if (n >= kProjMultiMin && nphi <= 2048)
+        return prec == AW_PREC_F64 ? launch_project_multi<double>(psi, phis, nphi, n, st)
+                                   : launch_project_multi<float>(psi, phis, nphi, n, st);
     const size_t shmem = sizeof(double) * 2 * (nphi > 0 ? nphi : 1);
     if (prec == AW_PREC_F64)
         hipLaunchKernelGGL(k_project_out<double>, dim3(1), dim3(RED_THREADS), shmem, st, psi, phis, nphi, n);

@@ -335,10 +335,14 @@ int aw_module_launch(aw_module *mod, const char *kernel, unsigned gx, unsigned g
 
 /*
  * Gram-Schmidt step of projectout_stationary_states (schrodinger_wigner_cuda_1d.py:505-550)
- * on one vector of n complex elements (one workgroup, FP64 sums):
+ * on one vector of n complex elements (FP64 sums):
  *   psi /= ||psi||;  c_k = sum conj(phi_k) psi;  psi -= sum_k c_k phi_k;  psi /= ||psi||.
  * phis holds nphi vectors of length n contiguously (unit 2-norm, like the reference's
- * stationary_states); n <= 65536.
+ * stationary_states); n <= 65536, nphi <= 4096.  Lines below 8192 points run in one
+ * workgroup; longer ones (nphi <= 2048) in three launches of 128 workgroups over slices of
+ * the line, whose partial sums live in a device buffer the library allocates on the first
+ * such call on a device (not under stream capture) and shares between calls: such calls on
+ * one device must not overlap.
  */
 int aw_project_out(int prec, void *psi, const void *phis, int nphi, int n, aw_stream stream);
 
