@@ -174,6 +174,28 @@ __device__ double aw_absp(double PP, double PP_prime, double t, size_t i, size_t
 {
     return (%s);
 }
+__device__ double aw_mask(int which, double xj, double xi, double t, size_t i, size_t j)
+{
+    return which == 0 ? aw_absx(xj, xi, t, i, j) : aw_absp(xj, xi, t, i, j);
+}
+// a separable mask M(t)[i][j] = v[i] u[j] at each of nt times: u[s][j] = M(t_s)[i0][j],
+// v[s][i] = M(t_s)[i][j0] / M(t_s)[i0][j0]
+extern "C" __global__ void aw_eval_mask_lines(int which, const double *xj, int nj, const double *xi, int ni,
+                                              const double *ts, int nt, int i0, int j0, double *u, double *v)
+{
+    const long long total = (long long)nt * (nj + ni);
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int s = (int)(idx / (nj + ni)), k = (int)(idx %% (nj + ni));
+        const double t = ts[s];
+        if (k < nj) {
+            u[(long long)s * nj + k] = aw_mask(which, xj[k], xi[i0], t, i0, k);
+        } else {
+            const int i = k - nj;
+            v[(long long)s * ni + i] = aw_mask(which, xj[j0], xi[i], t, i, j0) / aw_mask(which, xj[j0], xi[i0], t, i0, j0);
+        }
+    }
+}
 extern "C" __global__ void aw_eval_mask(int which, const double *xj, int nj, const double *xi,
                                         int ni, double t, double *out)
 {
@@ -210,9 +232,12 @@ extern "C" __global__ void aw_eval_mask(int which, const double *xj, int nj, con
         return out.get()
 
     def _prepare_masks(self):
-        """Evaluate abs_boundary_x(XX, XX') and abs_boundary_p(PP, PP') once (they are
-        applied in every expV / expK, :753, :777).  A separable mask becomes two vectors;
-        a non-separable one a 2D table.  Masks that depend on t are rejected."""
+        """Evaluate abs_boundary_x(XX, XX', t) and abs_boundary_p(PP, PP', t), which expV /
+        expK apply at the step time t (:753, :777).  A time-independent mask is evaluated once:
+        two vectors when it is separable, else a 2D table.  A time-dependent mask must stay
+        separable (rank 1 at every sampled time): its line and element vectors are evaluated
+        on the device for every step of a chunk (``_mask_tables``) and multiply the factor
+        vectors; a time-dependent mask that is not separable raises NotImplementedError."""
         self._masks = {}
         for which, expr in ((0, self.abs_boundary_x), (1, self.abs_boundary_p)):
             if expr.strip() in ("1.", "1", "1.0"):
@@ -221,8 +246,19 @@ extern "C" __global__ void aw_eval_mask(int which, const double *xj, int nj, con
             M0 = self._eval_mask(which, float(self.t))
             M1 = self._eval_mask(which, float(self.t) + 1.2345)
             if not np.array_equal(M0, M1):
-                raise NotImplementedError(
-                    "time-dependent absorbing boundaries are not supported (%r)" % expr)
+                M2 = self._eval_mask(which, float(self.t) + 2.7183)
+                if any(rank1_split(M) is None for M in (M0, M1, M2)):
+                    raise NotImplementedError(
+                        "time-dependent absorbing boundaries must be separable, f(XX, t) g(XX', t) "
+                        "(%r)" % expr)
+                # pivot of the split: the largest entry, where the mask is nearest 1
+                i0, j0 = np.unravel_index(np.argmax(np.abs(M0)), M0.shape)
+                if any(M[i0, j0] == 0.0 for M in (M1, M2)):
+                    raise NotImplementedError(
+                        "time-dependent absorbing boundary %r vanishes at its largest entry (%d, %d) "
+                        "at a later time" % (expr, i0, j0))
+                self._masks[which] = ("time", int(i0), int(j0))
+                continue
             split = rank1_split(M0)
             if split is not None:
                 v, u = split
@@ -500,9 +536,31 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             ts[k] = t
         return ts
 
+    def _tdep_masks(self):
+        return any(m is not None and m[0] == "time" for m in self._masks.values())
+
+    def _mask_tables(self, ts_dev, n, out=None):
+        """Per-step vectors of the time-dependent masks at the step times ts_dev[:n] (t after
+        the increment, the time expV / expK see): {which: (line [n][rows], elem [n][cols])},
+        written into ``out`` when given."""
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        tabs = {} if out is None else out
+        for which, m in self._masks.items():
+            if m is None or m[0] != "time":
+                continue
+            if which not in tabs or tabs[which][0].shape[0] < n:
+                tabs[which] = (empty((n, Np), np.float64), empty((n, Nx), np.float64))
+            v, u = tabs[which]
+            xj, xi = (self._dev_kXX, self._dev_kXXp) if which == 0 else (self._dev_kPP, self._dev_kPPp)
+            blocks, threads = rtc.launch_cfg(n * (Nx + Np))
+            self._mod.launch("aw_eval_mask_lines", (blocks,), (threads,),
+                             [ctypes.c_int(which), xj.ptr, ctypes.c_int(Nx), xi.ptr, ctypes.c_int(Np), ts_dev.ptr,
+                              ctypes.c_int(n), ctypes.c_int(m[1]), ctypes.c_int(m[2]), u.ptr, v.ptr], current_stream())
+        return tabs
+
     def _mask_args(self, which, power):
         m = self._masks.get(which)
-        if m is None:
+        if m is None or m[0] == "time":
             return None, None, None
         if m[0] == "vec":
             return m[1], m[2], None          # (v over lines-i, u over cols-j)
@@ -533,7 +591,10 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         self._eval_1d(0, self._dev_kXXp, Np, ts_dev, n, 0.5 * dt, Vi)
         self._eval_1d(1, self._dev_kPP, Nx, ts_dev, n, 0.5 * dt, Kj)
         self._eval_1d(1, self._dev_kPPp, Np, ts_dev, n, 0.5 * dt, Ki)
-        return self._factors_from_tables(n, Vj.ptr, Vi.ptr, Kj.ptr, Ki.ptr, dt, merged, dst=dst)
+        mask_t = None
+        if self._tdep_masks():
+            mask_t = self._mask_tables(ts_dev, n, None if bufs is None else bufs.setdefault("mask_t", {}))
+        return self._factors_from_tables(n, Vj.ptr, Vi.ptr, Kj.ptr, Ki.ptr, dt, merged, dst=dst, mask_t=mask_t)
 
     def _chunk_bufs(self, n):
         """Persistent per-chunk inputs of the per-step loops (fixed addresses for capture)."""
@@ -544,17 +605,22 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
                     fac=(p.new_factors(Nx, 2 * n), p.new_factors(Np, 2 * n),
                          p.new_factors(Np, n), p.new_factors(Nx, n)))
 
-    def _factors_from_tables(self, n, Vj, Vi, Kj, Ki, dt, merged=True, k_stride=None, dst=None):
+    def _factors_from_tables(self, n, Vj, Vi, Kj, Ki, dt, merged=True, k_stride=None, dst=None, mask_t=None):
         """Factor vectors of n steps from FP64 phase tables (device pointers) V(XX_j),
         V(XX'_i) [n, Nx] / [n, Np] and K(PP_j), K(PP'_i) (k_stride 0: one row for all
         steps).  A negative dt gives the conjugated factors: the adjoint step.  dst:
-        preallocated (colL, colE, rowL, rowE) with room for the vectors."""
+        preallocated (colL, colE, rowL, rowE) with room for the vectors.  mask_t: per-step
+        vectors of time-dependent masks (``_mask_tables``)."""
         Nx, Np = self.X_gridDIM, self.P_gridDIM
         p = self.plan
         mxp, mx, _ = self._mask_args(0, 1)
         mpp, mp, _ = self._mask_args(1, 1)
         mxp_p = mxp.ptr if mxp is not None else None
         mx_p = mx.ptr if mx is not None else None
+        mt = mask_t or {}
+        if 0 in mt or 1 in mt:
+            return self._factors_masks_t(n, Vj, Vi, Kj, Ki, dt, merged, k_stride, dst, mt, mx_p, mxp_p,
+                                         mpp.ptr if mpp is not None else None, mp.ptr if mp is not None else None)
         if dst is not None:
             colL, colE, rowL, rowE = dst
         else:
@@ -588,6 +654,63 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         p.factors(colL, Nx, 1, A=Vj + (n - 1) * Nx * 8, ca=-0.5 * dt, mag=mx_p, alt_sign=True,
                   out_offset=n)
         p.factors(colE, Np, 1, A=Vi + (n - 1) * Np * 8, ca=0.5 * dt, mag=mxp_p, alt_sign=True,
+                  scale=1.0 / Np, out_offset=n)
+        return colL, colE, rowL, rowE
+
+    def _factors_masks_t(self, n, Vj, Vi, Kj, Ki, dt, merged, k_stride, dst, mt, mx_p, mxp_p, mpp_p, mp_p):
+        """_factors_from_tables with at least one time-dependent mask: the step's mask vectors
+        (mag, one per step, stride one vector) multiply its factor vectors; a merged column
+        vector takes the closing step's mask times the opening step's."""
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        p = self.plan
+
+        def mags(which, static_line, static_elem, nl, ne):
+            """(line mag, line stride, elem mag, elem stride) of n per-step vectors."""
+            if which in mt:
+                v, u = mt[which]
+                return v, nl, u, ne
+            return static_line, 0, static_elem, 0
+
+        pl, pls, pe, pes = mags(1, mpp_p, mp_p, Np, Nx)
+        xl, xls, xe, xes = mags(0, mxp_p, mx_p, Np, Nx)
+        ptr = lambda a: a.ptr if isinstance(a, DeviceArray) else a   # noqa: E731
+        if dst is not None:
+            colL, colE, rowL, rowE = dst
+        else:
+            rowL, rowE = p.new_factors(Np, n), p.new_factors(Nx, n)
+        p.factors(rowL, Np, n, A=Ki, ca=dt, mag=ptr(pl), mag_stride=pls, scale=1.0 / Nx, a_stride=k_stride)
+        p.factors(rowE, Nx, n, A=Kj, ca=-dt, mag=ptr(pe), mag_stride=pes, a_stride=k_stride)
+        if not merged:
+            if dst is None:
+                colL, colE = p.new_factors(Nx, 2 * n), p.new_factors(Np, 2 * n)
+            for v, scale in ((0, 1.0), (1, 1.0 / Np)):
+                p.factors(colL, Nx, n, A=Vj, ca=-0.5 * dt, mag=ptr(xe), mag_stride=xes, alt_sign=True,
+                          out_offset=v, out_vstride=2)
+                p.factors(colE, Np, n, A=Vi, ca=0.5 * dt, mag=ptr(xl), mag_stride=xls, alt_sign=True, scale=scale,
+                          out_offset=v, out_vstride=2)
+            return colL, colE, rowL, rowE
+        if dst is None:
+            colL, colE = p.new_factors(Nx, n + 1), p.new_factors(Np, n + 1)
+
+        def at(a, s, size):
+            return None if a is None else ptr(a) + (s * size * 8 if isinstance(a, DeviceArray) else 0)
+
+        p.factors(colL, Nx, 1, A=Vj, ca=-0.5 * dt, mag=at(xe, 0, Nx), alt_sign=True)
+        p.factors(colE, Np, 1, A=Vi, ca=0.5 * dt, mag=at(xl, 0, Np), alt_sign=True)
+        if n > 1:
+            # merged kicks: the closing step's mask times the opening step's
+            if isinstance(xe, DeviceArray):
+                me = DeviceArray(None, np.float64, tensor=xe.tensor[:n - 1] * xe.tensor[1:n])
+                ml = DeviceArray(None, np.float64, tensor=xl.tensor[:n - 1] * xl.tensor[1:n])
+                kw_e, kw_l = dict(mag=me.ptr, mag_stride=Nx), dict(mag=ml.ptr, mag_stride=Np)
+            else:
+                kw_e, kw_l = dict(mag=xe, mag_pow=2), dict(mag=xl, mag_pow=2)
+            p.factors(colL, Nx, n - 1, A=Vj, ca=-0.5 * dt, B=Vj + Nx * 8, cb=-0.5 * dt, out_offset=1, **kw_e)
+            p.factors(colE, Np, n - 1, A=Vi, ca=0.5 * dt, B=Vi + Np * 8, cb=0.5 * dt, scale=1.0 / Np, out_offset=1,
+                      **kw_l)
+        p.factors(colL, Nx, 1, A=Vj + (n - 1) * Nx * 8, ca=-0.5 * dt, mag=at(xe, n - 1, Nx), alt_sign=True,
+                  out_offset=n)
+        p.factors(colE, Np, 1, A=Vi + (n - 1) * Np * 8, ca=0.5 * dt, mag=at(xl, n - 1, Np), alt_sign=True,
                   scale=1.0 / Np, out_offset=n)
         return colL, colE, rowL, rowE
 
@@ -675,7 +798,8 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
                              graphs={})
             self._observable_kernel(exprs)
             self._obs_partials_for(nobs)             # allocated here, never inside a capture
-            g = self._obs_weights(exprs) if self.obs_by_adjoint else None
+            # (time-dependent masks change the closing half-kick every step: no merged steps)
+            g = self._obs_weights(exprs) if (self.obs_by_adjoint and not self._tdep_masks()) else None
             # time-independent observables imply a time-independent V (the Hamiltonian is one of
             # them): the steps are merged and the observables read from each row-pass output
             self._ehr["weights"] = self._closing_adjoint(g) if g is not None else None

@@ -150,17 +150,24 @@ class WignerMoyalOracle:
             self.hamiltonian_average = []
 
     # -- elementwise kernels ------------------------------------------------------------
-    def _absx(self):
+    @staticmethod
+    def _mask(f, a, b, t):
+        """A mask callable f(XX, XX') or f(XX, XX', t): the reference inlines the mask's C
+        expression into expV / expK, where the step time t is in scope (:753, :777)."""
+        import inspect
+        if len(inspect.signature(f).parameters) >= 3:
+            return f(a, b, t)
+        return f(a, b)
+
+    def _absx(self, t=0.0):
         if self.abs_boundary_x is None:
             return 1.0
-        return _as_grid(self.abs_boundary_x(self.kXX[None, :], self.kXXp[:, None]),
-                        self.rho.shape)
+        return _as_grid(self._mask(self.abs_boundary_x, self.kXX[None, :], self.kXXp[:, None], t), self.rho.shape)
 
-    def _absp(self):
+    def _absp(self, t=0.0):
         if self.abs_boundary_p is None:
             return 1.0
-        return _as_grid(self.abs_boundary_p(self.kPP[None, :], self.kPPp[:, None]),
-                        self.rho.shape)
+        return _as_grid(self._mask(self.abs_boundary_p, self.kPP[None, :], self.kPPp[:, None], t), self.rho.shape)
 
     def expV_factor(self, t):
         """expV kernel factor (wigner_moyal_cuda_1d.py:763-778)."""
@@ -168,7 +175,7 @@ class WignerMoyalOracle:
         Vj = _as_grid(self.V(self.kXX, tt), self.kXX.shape)
         Vi = _as_grid(self.V(self.kXXp, tt), self.kXXp.shape)
         phase = -0.5 * self.dt * (Vj[None, :] - Vi[:, None])
-        return self.S * np.exp(1j * phase) * self._absx()
+        return self.S * np.exp(1j * phase) * self._absx(t)
 
     def expK_factor(self, t):
         """expK kernel factor (wigner_moyal_cuda_1d.py:742-754)."""
@@ -176,7 +183,7 @@ class WignerMoyalOracle:
         Kj = _as_grid(self.K(self.kPP, tt), self.kPP.shape)
         Ki = _as_grid(self.K(self.kPPp, tt), self.kPPp.shape)
         phase = -self.dt * (Kj[None, :] - Ki[:, None])
-        return np.exp(1j * phase) * self._absp()
+        return np.exp(1j * phase) * self._absp(t)
 
     def shearX_factor(self):
         """phase_shearX (wigner_moyal_cuda_1d.py:639-654)."""

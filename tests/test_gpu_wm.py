@@ -121,6 +121,43 @@ def test_wm_time_dependent_and_masks():
     assert rel(g.propagate(150).get(), o.propagate(150)) < 1e-10
 
 
+def _tmask_pair(ehrenfest):
+    mx = "pow(fabs(cos(0.5 * M_PI * XX / 20.) * cos(0.5 * M_PI * XX_prime / 30.)), 0.02 * (1. + sin(t)))"
+    mp = "exp(-0.001 * (1. + t) * (PP * PP + PP_prime * PP_prime))"
+    fx = lambda XX, XXp, t: np.abs(np.cos(0.5 * np.pi * XX / 20.) * np.cos(0.5 * np.pi * XXp / 30.)) ** (0.02 * (1. + np.sin(t)))  # noqa: E731,E501
+    fp = lambda PP, PPp, t: np.exp(-0.001 * (1. + t) * (PP * PP + PPp * PPp))  # noqa: E731
+    V = "0.5 * X * X + 0.1 * X * sin(t)"
+    Vf = lambda X, t: 0.5 * X * X + 0.1 * X * np.sin(t)  # noqa: E731
+    g, o = _pair(Nx=128, Np=64, Xa=8., Pa=6., V=V, Vf=Vf, ehrenfest=ehrenfest, abs_boundary_x=mx, abs_boundary_p=mp)
+    o.abs_boundary_x, o.abs_boundary_p = fx, fp
+    g.set_wignerfunction("exp(-pow(X - 1., 2) - pow(P - 0.5, 2)) / M_PI")
+    o.set_wignerfunction(lambda X, P: np.exp(-(X - 1.) ** 2 - (P - 0.5) ** 2) / np.pi)
+    return g, o
+
+
+@pytest.mark.parametrize("ehrenfest", [False, True])
+def test_wm_time_dependent_separable_masks(ehrenfest):
+    """Absorbing masks that depend on the step time (the reference evaluates the mask
+    expression inside expV / expK with t in scope, wigner_moyal_cuda_1d.py:753, :777):
+    per-step line and element vectors, merged steps and the Ehrenfest loop vs the oracle."""
+    g, o = _tmask_pair(ehrenfest)
+    assert g._tdep_masks()
+    W = g.propagate(300 if not ehrenfest else 60).get()
+    Wo = o.propagate(300 if not ehrenfest else 60)
+    assert rel(W, Wo) < 1e-10
+    if ehrenfest:
+        for name in ("X_average", "P_average", "X_average_RHS", "P_average_RHS", "hamiltonian_average"):
+            a, b = np.array(getattr(g, name)), np.array(getattr(o, name))
+            assert np.max(np.abs(a - b)) < 1e-10 * max(1.0, np.max(np.abs(b))), name
+
+
+def test_wm_time_dependent_nonseparable_mask_is_refused():
+    from accuratewigner_amd import WignerMoyalCUDA1D
+    with pytest.raises(NotImplementedError):
+        WignerMoyalCUDA1D(X_gridDIM=64, X_amplitude=5., P_gridDIM=64, P_amplitude=5., dt=0.01, t=0.,
+                          K="0.5 * P * P", V="0.5 * X * X", abs_boundary_x="1. - 0.001 * t * XX * XX_prime")
+
+
 def test_wm_nonseparable_mask_table():
     ap = "(abs(PP + PP_prime) < 5. ? 1. : 0.98)"
     g, o = _pair(Nx=128, Np=64, abs_boundary_p=ap)

@@ -124,3 +124,20 @@ def test_cufft_plan_many_rejects_real_types_without_a_gpu_call():
     for t in (cufft.CUFFT_R2C, cufft.CUFFT_C2R, cufft.CUFFT_D2Z, cufft.CUFFT_Z2D):
         with __import__("pytest").raises(cufft.cufftInvalidType):
             cufft.cufftPlanMany(1, np.array([64], np.int32).ctypes.data, None, 1, 0, None, 1, 0, t, 1)
+
+
+def test_wm_physics_source_with_time_dependent_masks_compiles():
+    """The Wigner-Moyal physics module (V, K tables, mask tables and per-step mask lines) with
+    mask expressions in XX, XX', PP, PP' and t compiles with hipRTC (no GPU needed)."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libawig.so not built")
+    import types
+    from accuratewigner_amd import rtc
+    from accuratewigner_amd.wigner_moyal_cuda_1d import WignerMoyalCUDA1D
+    ns = types.SimpleNamespace(
+        cuda_consts=rtc.cuda_consts(dict(X_gridDIM=64, P_gridDIM=64), None, None),
+        K="0.5 * P * P", V="0.5 * X * X + 0.1 * X * sin(t)",
+        abs_boundary_x="pow(fabs(cos(0.5 * M_PI * XX / 20.) * cos(0.5 * M_PI * XX_prime / 30.)), 0.02 * (1. + sin(t)))",
+        abs_boundary_p="exp(-0.001 * (1. + t) * (PP * PP + PP_prime * PP_prime))")
+    ok, log = rtc.check_source(WignerMoyalCUDA1D._physics_source(ns))
+    assert ok, log
