@@ -138,6 +138,7 @@ class WignerMoyalCUDA1D(KeywordParams):
         self._ehr = None          # persistent chunk buffers and captured graphs of the Ehrenfest loop
         self._prop = None         # the same for the merged-step loop
         self._prepare_masks()
+        self._static = self._h_static()
         self.print_memory_info()
 
     # ------------------------------------------------------------------------------------
@@ -566,7 +567,39 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             return m[1], m[2], None          # (v over lines-i, u over cols-j)
         return None, None, (m[1] if power == 1 else m[2])
 
+    def _h_static(self):
+        """True when V, K and the masks cannot depend on t: no token ``t`` in the V and K
+        expressions nor in a macro of the constants and functions, and no time-dependent mask.
+        The factor vectors of every step are then the same and are built once."""
+        return (not self._tdep_masks() and time_independent(self.V, self.cuda_consts)
+                and time_independent(self.K, self.cuda_consts))
+
     def _build_chunk_factors(self, ts, merged=True, bufs=None):
+        """The factor vectors of one chunk (``_build_factors``); with a time-independent H and
+        persistent buffers, the constant vectors of any chunk, built on first use: merged
+        open / mid / close column vectors 0, 1, 2 and row vector 0, or one unmerged step's open
+        and close column vectors 0, 1 and row vector 0."""
+        if not (self._static and bufs is not None):
+            return self._build_factors(ts, merged, bufs)
+        key = "merged" if merged else "split"
+        if bufs.get("static") != key:
+            n = 2 if merged else 1
+            self._build_factors(np.full(n, float(self.t) + self.dt), merged, bufs)
+            bufs["static"] = key
+        if ts is not None:
+            self._stage_times(bufs, ts)    # the step times, for observables that read t
+        return bufs["fac"]
+
+    @staticmethod
+    def _stage_times(bufs, ts):
+        ts_dev, n = bufs["ts"], len(ts)
+        if isinstance(ts, np.ndarray):
+            ts_dev[:n] = ts
+        else:
+            ts_dev.tensor[:n].copy_(ts)   # a device slice: ordered on the stream, no host wait
+        return ts_dev
+
+    def _build_factors(self, ts, merged=True, bufs=None):
         """Phase tables (FP64, hipRTC) and the per-pass complex factor vectors of one chunk.
         merged: the closing half-kick of a step and the opening one of the next share one
         column pass (n + 1 column vectors); otherwise each step has its own opening and
@@ -580,11 +613,7 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             Kj, Ki = empty((n, Nx), np.float64), empty((n, Np), np.float64)
             dst = None
         else:
-            ts_dev = bufs["ts"]
-            if isinstance(ts, np.ndarray):
-                ts_dev[:n] = ts
-            else:
-                ts_dev.tensor[:n].copy_(ts)   # a device slice: ordered on the stream, no host wait
+            ts_dev = self._stage_times(bufs, ts)
             Vj, Vi, Kj, Ki = bufs["Vj"], bufs["Vi"], bufs["Kj"], bufs["Ki"]
             dst = bufs["fac"]
         self._eval_1d(0, self._dev_kXX, Nx, ts_dev, n, 0.5 * dt, Vj)
@@ -752,7 +781,7 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         ts = self._times(steps)
         # the step times of the whole call in one upload; each chunk copies its slice on the
         # device (a host upload per chunk would wait for the chunk before it)
-        ts_all = to_device(ts)
+        ts_all = None if (self._static and not self.isEhrenfest) else to_device(ts)
         if not self.isEhrenfest:
             self._flush_scale()
             CH = CHUNK_STEPS
@@ -763,7 +792,8 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             normalized = False
             for c0 in range(0, steps, CH):
                 n = min(CH, steps - c0)
-                self._build_chunk_factors(ts_all.tensor[c0:c0 + n], merged=True, bufs=self._prop)
+                self._build_chunk_factors(None if ts_all is None else ts_all.tensor[c0:c0 + n], merged=True,
+                                          bufs=self._prop)
                 groups = self._item_groups(n) if (c0 + n == steps and self.obs_by_adjoint) else None
                 if groups:
                     # the last chunk of a batch that outgrows the MALL: each group of items runs
@@ -849,7 +879,7 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         _, _, tab2 = self._mask_args(0, 2)
         _, _, tabp = self._mask_args(1, 1)
         self.plan.steps(self.rho.ptr + b0 * self._item_bytes(), n, colL, colE, rowL, rowE, mask_open_close=tab1,
-                        mask_mid=tab2, mask_row=tabp, batch=nb)
+                        mask_mid=tab2, mask_row=tabp, batch=nb, const_steps=self._static)
 
     def _item_bytes(self):
         return self.X_gridDIM * self.P_gridDIM * np.dtype(self.state_dtype).itemsize
@@ -903,8 +933,10 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         stage, ts = bufs["stage"], bufs["ts"]
         for k in range(n):
             # one step = three passes; its opening / closing vectors sit at 2 k and 2 k + 1
-            p.steps(rho, 1, colL.ptr + 2 * k * Nx * csz, colE.ptr + 2 * k * Np * csz,
-                    rowL.ptr + k * Np * csz, rowE.ptr + k * Nx * csz, mask_open_close=tab1,
+            # (at 0 and 1 for every step with a time-independent H)
+            kc = 0 if self._static else k
+            p.steps(rho, 1, colL.ptr + 2 * kc * Nx * csz, colE.ptr + 2 * kc * Np * csz,
+                    rowL.ptr + kc * Np * csz, rowE.ptr + kc * Nx * csz, mask_open_close=tab1,
                     mask_row=tabp, dscale=self._dscale)
             self._scale_pending = False
             self.rho2wigner_()
@@ -935,12 +967,15 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         ring_row = self.batch * nblk * nobs * 8
         p.run(rho, 0, 0, INV, line_fac=colL, elem_fac=colE, mask=tab1)
         for k in range(n):
-            # the row pass also leaves the block partials of Re <g', Y_k> in the ring
-            p.run(rho, 1, FWD, INV, line_fac=rowL.ptr + k * Np * csz, elem_fac=rowE.ptr + k * Nx * csz,
-                  mask=tabp, obs_weights=gy, obs_nw=nobs, obs_partials=ring.ptr + k * ring_row)
             last = k == n - 1
-            p.run(rho, 0, FWD, 0 if last else INV, line_fac=colL.ptr + (k + 1) * Nx * csz,
-                  elem_fac=colE.ptr + (k + 1) * Np * csz, mask=tab1 if last else tab2)
+            # vectors of step k: row k and column k + 1 (row 0 and column 1, or 2 when closing,
+            # with a time-independent H)
+            kr, kc = (0, 2 if last else 1) if self._static else (k, k + 1)
+            # the row pass also leaves the block partials of Re <g', Y_k> in the ring
+            p.run(rho, 1, FWD, INV, line_fac=rowL.ptr + kr * Np * csz, elem_fac=rowE.ptr + kr * Nx * csz,
+                  mask=tabp, obs_weights=gy, obs_nw=nobs, obs_partials=ring.ptr + k * ring_row)
+            p.run(rho, 0, FWD, 0 if last else INV, line_fac=colL.ptr + kc * Nx * csz,
+                  elem_fac=colE.ptr + kc * Np * csz, mask=tab1 if last else tab2)
         finish_sums(ring, nblk, nobs, n * self.batch, bufs["stage"])
 
     def _closing_adjoint(self, g):

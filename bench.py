@@ -13,6 +13,7 @@ Prints ONE JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W
 """
 import argparse
 import contextlib
+import gc
 import json
 import os
 import sys
@@ -26,6 +27,19 @@ sys.path.insert(0, ROOT)
 METRIC = "grid-point-steps/s vs plain FP64 at stated error; p50/p99 solve latency"
 HBM_PEAK_GBPS = 8000.0
 HEADLINE_PRECISION = "mixed"
+
+
+@contextlib.contextmanager
+def no_gc():
+    """Python's cyclic garbage collector off inside a timed region (as timeit does): a full
+    collection of the interpreter's heap takes tens of ms and lands in whichever region it
+    falls, which is host noise, not the work being measured."""
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        gc.enable()
 
 
 def make_system(N, batch, precision, L=10.0, dt=0.01):
@@ -50,10 +64,11 @@ def time_propagate(sysm, warmup, steps):
     sysm.propagate(warmup)
     torch.cuda.synchronize()
     sysm._pass_events = []
-    t0 = time.perf_counter()
-    sysm.propagate(steps)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    with no_gc():
+        t0 = time.perf_counter()
+        sysm.propagate(steps)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
     evs, sysm._pass_events = sysm._pass_events, None
     pass_ms = sum(a.elapsed_time(b) for a, b, _ in evs)
     npass = sum(n for _, _, n in evs)
@@ -192,10 +207,11 @@ def leg_latency():
             getattr(c1, name).clear()
         c1.set_wignerfunction("exp(-pow(X - 2., 2) - P * P) / M_PI")
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        c1.propagate(1000)
-        torch.cuda.synchronize()
-        lat.append((time.perf_counter() - t0) * 1e3)
+        with no_gc():
+            t0 = time.perf_counter()
+            c1.propagate(1000)
+            torch.cuda.synchronize()
+            lat.append((time.perf_counter() - t0) * 1e3)
     lat = np.array(lat[2:])
     lat_nr = []
     one = make_system(256, 1, "mixed")
@@ -203,10 +219,11 @@ def leg_latency():
     for r in range(12):
         one.set_wignerfunction(st)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        one.propagate(1000)
-        torch.cuda.synchronize()
-        lat_nr.append((time.perf_counter() - t0) * 1e3)
+        with no_gc():
+            t0 = time.perf_counter()
+            one.propagate(1000)
+            torch.cuda.synchronize()
+            lat_nr.append((time.perf_counter() - t0) * 1e3)
     lat_nr = np.array(lat_nr[2:])
     return {"workload": "config 1 solve: WM FP64 256x256, 1000 steps, norm/<x>/<p>/<H> every step "
                         "(20 timed solves)",
@@ -230,10 +247,11 @@ def leg_gibbs():
         times = []
         for r in range(6):
             torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            wb.get_gibbs_state(record=[wb.hamiltonian])
-            torch.cuda.synchronize()
-            times.append((time.perf_counter() - t0) * 1e3)
+            with no_gc():
+                t0 = time.perf_counter()
+                wb.get_gibbs_state(record=[wb.hamiltonian])
+                torch.cuda.synchronize()
+                times.append((time.perf_counter() - t0) * 1e3)
         h_rec[prec] = wb.gibbs_record[:, 0].copy()
         p50 = float(np.percentile(times[1:], 50))
         gibbs[prec] = {"p50_ms": p50, "p99_ms": float(np.percentile(times[1:], 99)),
@@ -268,10 +286,11 @@ def leg_strong(max_steps=0):
         for name in ("X_average", "P_average", "X_average_RHS", "P_average_RHS", "hamiltonian_average"):
             getattr(g3, name).clear()
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        g3.propagate(steps3)
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
+        with no_gc():
+            t0 = time.perf_counter()
+            g3.propagate(steps3)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
         accel[prec] = np.array(g3.P_average_RHS)
         strong[prec] = {"s": el, "us_per_step": el / steps3 * 1e6, "gp_steps_per_s": 1024 ** 2 * steps3 / el}
         del g3
@@ -295,11 +314,12 @@ def leg_adjoint():
     times = []
     for r in range(3):
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        c4.forward(E4)
-        c4.gradient()
-        torch.cuda.synchronize()
-        times.append((time.perf_counter() - t0) * 1e3)
+        with no_gc():
+            t0 = time.perf_counter()
+            c4.forward(E4)
+            c4.gradient()
+            torch.cuda.synchronize()
+            times.append((time.perf_counter() - t0) * 1e3)
     return {"workload": "config 4: forward + adjoint gradient, WM FP64 512x512, 2000 steps, 64 pieces, "
                         "checkpoint 50 (3 timed runs)",
             "p50_ms": float(np.percentile(times, 50)), "max_ms": float(np.max(times))}

@@ -278,3 +278,29 @@ def test_batched_2048_grid_mixed_matches_fp64():
         del s
     assert out["fp64"].shape == (2, 2048, 2048)
     assert rel(out["mixed"], out["fp64"]) < 2e-5
+
+
+@pytest.mark.parametrize("ehrenfest", [False, True])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_wm_static_factors_equal_per_step_factors(ehrenfest, prec):
+    """With V and K free of t the factor vectors are built once and reused by every chunk
+    (constant-step sequences); the result equals the per-step factor path bit for bit over
+    several chunks and calls, masks included."""
+    def run(static):
+        g, _ = _pair(Nx=128, Np=64, Xa=8., Pa=6., V="0.5 * X * X + 0.01 * X * X * X * X",
+                     ehrenfest=ehrenfest, precision=prec,
+                     abs_boundary_x="pow(fabs(cos(0.5 * M_PI * XX / 20.)), 0.01)")
+        assert g._static
+        g._static = static
+        g.set_wignerfunction("exp(-pow(X - 1., 2) - pow(P - 0.5, 2)) / M_PI")
+        g.propagate(300)
+        W = g.propagate(7).get()
+        lists = [np.array(getattr(g, n)) for n in ("X_average", "P_average", "hamiltonian_average")] \
+            if ehrenfest else []
+        return W, lists
+    Ws, Ls = run(True)
+    Wd, Ld = run(False)
+    assert np.array_equal(Ws, Wd)
+    for a, b in zip(Ls, Ld):
+        assert np.array_equal(a, b)
+
