@@ -86,6 +86,7 @@ class aw_pass(ctypes.Structure):
         ("obs_weights", ctypes.c_void_p), ("obs_nw", ctypes.c_int), ("obs_partials", ctypes.c_void_p),
         ("obs_mode", ctypes.c_int), ("obs_norm2", ctypes.c_int), ("obs_norm2_weights", ctypes.c_void_p),
         ("in_scaled_out", ctypes.c_void_p),
+        ("obs_line_fac", ctypes.c_void_p), ("obs_elem_fac", ctypes.c_void_p), ("obs_diag_nw", ctypes.c_int),
     ]
 
 

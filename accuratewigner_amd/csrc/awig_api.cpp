@@ -176,11 +176,29 @@ static int pass_run_n(aw_plan *p, void *data, const aw_pass *ps, int nbatch, aw_
     a.ctab = ps->ctab;
     a.c_bstride = ps->ctab_bstride;
     a.chirp_out = ps->chirp_out;
-    if (ps->obs_weights) {
+    if (ps->obs_mode == 2) {
+        // von Neumann observables of the state between two steps, from a column pass
+        if (!ps->obs_partials) return set_error(AW_ERR_INVALID_VALUE, "obs_mode 2 needs an obs_partials buffer");
+        if (!strided || p->rows != p->cols)
+            return set_error(AW_ERR_INVALID_VALUE, "obs_mode 2 needs a column pass over square items");
+        if (ps->obs_nw < 0 || ps->obs_diag_nw < 0 || 1 + ps->obs_nw + ps->obs_diag_nw > 8)
+            return set_error(AW_ERR_INVALID_VALUE, "obs_mode 2: 1 + obs_nw + obs_diag_nw must be at most 8");
+        if ((ps->obs_nw > 0 && !ps->obs_weights) || (ps->obs_diag_nw > 0 && !ps->obs_norm2_weights))
+            return set_error(AW_ERR_INVALID_VALUE, "obs_mode 2: NULL weights");
+        if (ps->obs_norm2) return set_error(AW_ERR_INVALID_VALUE, "obs_mode 2 takes no obs_norm2");
+        a.obs_w = static_cast<const double2 *>(ps->obs_weights);
+        a.obs_nw = ps->obs_nw;
+        a.obs_partials = ps->obs_partials;
+        a.obs_mode = 2;
+        a.obs_n2w = ps->obs_norm2_weights;
+        a.obs_nd = ps->obs_diag_nw;
+        a.obs_lf = ps->obs_line_fac;
+        a.obs_ef = ps->obs_elem_fac;
+    } else if (ps->obs_weights) {
         if (ps->obs_nw < 1 || ps->obs_nw > 8 || !ps->obs_partials)
             return set_error(AW_ERR_INVALID_VALUE, "obs_nw must be 1..8 with an obs_partials buffer");
         if (ps->obs_mode != 0 && ps->obs_mode != 1)
-            return set_error(AW_ERR_INVALID_VALUE, "obs_mode must be 0 or 1 (got %d)", ps->obs_mode);
+            return set_error(AW_ERR_INVALID_VALUE, "obs_mode must be 0, 1 or 2 (got %d)", ps->obs_mode);
         if (ps->obs_norm2 != 0 && ps->obs_norm2 != 1)
             return set_error(AW_ERR_INVALID_VALUE, "obs_norm2 must be 0 or 1 (got %d)", ps->obs_norm2);
         if (ps->obs_nw + ps->obs_norm2 > 8)

@@ -69,7 +69,12 @@ int launch_pass(int prec, int n, bool strided, const PassArgs &a, hipStream_t st
         if (a.dir_a == AW_FORWARD && a.dir_b == AW_INVERSE) kind = 7;
         else if (a.dir_a == AW_INVERSE && a.dir_b == 0) kind = 8;
     }
-    if (a.obs_w) {
+    if (a.obs_mode == 2) {
+        if (!strided || (kind != 2 && kind != 3))
+            return set_error(AW_ERR_INVALID_VALUE,
+                             "von Neumann observables need a plain [forward, factor(, inverse)] column pass");
+        kind = kind == 2 ? 9 : 10;
+    } else if (a.obs_w) {
         if (a.obs_mode == 0) {
             if (kind != 2 || strided)
                 return set_error(AW_ERR_INVALID_VALUE,

@@ -39,7 +39,11 @@ struct PassArgs {
     int obs_norm2;          // 1: one more component, sum |v|^2 (times obs_n2w when given)
     const double *obs_n2w;  // real weights [rows][cols] of that component, or NULL
     int obs_mode;           // 0: partials of the pass output (row passes); 1: of the values
-                            // right after the factor multiply (column passes)
+                            // right after the factor multiply (column passes); 2: von Neumann
+                            // observables of the state after the first FFT and the closing
+                            // vectors obs_lf / obs_ef (column passes of square items)
+    const void *obs_lf, *obs_ef;
+    int obs_nd;             // obs_mode 2: diagonal weights (obs_n2w rows)
     void *aux_out;          // general passes: the loaded input times dscale[b], stored here
 };
 

@@ -128,6 +128,9 @@ int launch_cfg(const PassArgs &a, hipStream_t st) {
 //         its output;
 //       5 / 6 column step [forward, factor, inverse] / [forward, factor] that stores the block
 //         partials of the values right after the factor multiply;
+//       9 / 10 column step [forward, factor, inverse] / [forward, factor] that stores the
+//         block partials of the von Neumann observables of the values after the first FFT
+//         times the closing vectors of a step (the state between two steps);
 //       7 / 8 general pass with the directions [forward, inverse] / [inverse, none] fixed (the
 //         passes of the Wigner transform: one FFT body per slot instead of both directions,
 //         104 -> 92 us for an FP32 1024-point row pass over 24 1024^2 states); the latency
@@ -149,6 +152,13 @@ int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
                 }
                 return set_error(AW_ERR_INVALID_VALUE,
                                  "partials after the factor: column passes of whole waves only");
+            case 9:
+            case 10:
+                if constexpr (STRIDED && PassKS<R, N, STRIDED, -1, 1, 0>::THREADS % 64 == 0) {
+                    if (kind == 9) return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0, 3, S>>(a, st);
+                    return launch_cfg<PassKS<R, N, STRIDED, -1, 0, 0, 3, S>>(a, st);
+                }
+                return set_error(AW_ERR_INVALID_VALUE, "von Neumann observables: column passes of whole waves only");
             case 7:
             case 8:
             case 1: if (kind == 1) return launch_cfg<PassKS<R, N, STRIDED, 0, 1, 0, 0, S>>(a, st);
@@ -170,6 +180,13 @@ int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
             return launch_cfg<PassK<R, N, STRIDED, -1, 0, 0, 2, S>>(a, st);
         }
         return set_error(AW_ERR_INVALID_VALUE, "partials after the factor: column passes of whole waves only");
+    case 9:
+    case 10:
+        if constexpr (STRIDED && PassK<R, N, STRIDED, -1, 1, 0>::THREADS % 64 == 0) {
+            if (kind == 9) return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0, 3, S>>(a, st);
+            return launch_cfg<PassK<R, N, STRIDED, -1, 0, 0, 3, S>>(a, st);
+        }
+        return set_error(AW_ERR_INVALID_VALUE, "von Neumann observables: column passes of whole waves only");
     case 7: return launch_cfg<PassK<R, N, STRIDED, -1, 1, 1, 0, S>>(a, st);
     case 8: return launch_cfg<PassK<R, N, STRIDED, 1, 0, 1, 0, S>>(a, st);
     case 1: return launch_cfg<PassK<R, N, STRIDED, 0, 1, 0, 0, S>>(a, st);

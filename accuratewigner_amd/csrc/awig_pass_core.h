@@ -278,7 +278,8 @@ struct KCfg {
     using S = S_;   // the state in HBM (converted on load and store)
     static constexpr int N = N_, E = E_, LPB = LPB_, TWM = TWM_, DA = DA_, DB = DB_, GEN = GEN_;
     static constexpr int OBS = OBS_;     // weighted-sum block partials of the output (1) or of the
-                                         // values right after the factor multiply (2)
+                                         // values right after the factor multiply (2); von Neumann
+                                         // observables of the closed step (3, columns)
     static constexpr int XS = XS_;       // 1: LDS exchange in real and imaginary halves
     static constexpr int MINW = MINW_;   // minimum waves per SIMD the register budget must allow
     static constexpr int PF = PF_;       // 1: load the factor vectors with the data, before the first FFT;
@@ -301,11 +302,40 @@ struct KCfg {
 constexpr int kObsPre = 4;
 template <typename C> constexpr int obs_prefetch() { return (C::OBS == 1 && C::E <= 4) ? kObsPre : 0; }
 
+// Adds each of the nc (uniform) components of acc over the block, in a fixed order (the
+// lanes of each wave, then the waves), into obs_partials[item][block][component].
+template <typename C>
+__device__ __forceinline__ void store_block_partials(const double (&acc)[8], int nc, const PassArgs &a, long long blk,
+                                                     int nl, int tid) {
+    constexpr int LPB = C::LPB;
+    static_assert(C::THREADS % 64 == 0, "block partials of whole waves");
+    __shared__ double ored[C::THREADS / 64][8];
+    const int lane = tid & 63, wid = tid >> 6;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        if (q < nc) {   // nc is uniform: only the components in use are reduced
+            double s = acc[q];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+            if (lane == 0) ored[wid][q] = s;
+        }
+    }
+    __syncthreads();
+    if (tid < nc) {
+        double s = 0.0;
+        for (int q = 0; q < C::THREADS / 64; ++q) s += ored[q][tid];
+        const long long g0 = blk * LPB;
+        const long long bb = g0 / nl;
+        const long long bi = (g0 - bb * nl) / LPB;
+        a.obs_partials[(bb * (nl / LPB) + bi) * nc + tid] = s;
+    }
+}
+
 template <typename C, typename T2>
 __device__ __forceinline__ void obs_block_partials(const PassArgs &a, const T2 *v, bool active, int off0,
                                                    int kstep, long long blk, int nl, int tid,
                                                    const double2 (*wpre)[C::E] = nullptr) {
-    constexpr int N = C::N, E = C::E, LPB = C::LPB;
+    constexpr int N = C::N, E = C::E;
     constexpr int NPRE = obs_prefetch<C>();
     static_assert(C::THREADS % 64 == 0, "weighted-sum partials of whole waves");
     const int nw = a.obs_nw, nc = a.obs_nw + a.obs_norm2;
@@ -335,26 +365,53 @@ __device__ __forceinline__ void obs_block_partials(const PassArgs &a, const T2 *
             }
         }
     }
-    __shared__ double ored[C::THREADS / 64][8];
-    const int lane = tid & 63, wid = tid >> 6;
+    store_block_partials<C>(acc, nc, a, blk, nl, tid);
+}
+
+// Block partials of the von Neumann observables of a square item's state between two steps,
+// from a column pass (line j = column j, element e = row i): the values right after the
+// first FFT times the step's closing vectors, z(i, j) = v * (obs_lf[j] * obs_ef[i]) (the same
+// product as the closing pass of an unmerged step), give
+//   [0] Re z(j, j), [1 + q] Re z(j, j) obs_n2w[q][j]  (on the diagonal element of the line),
+//   [1 + nd + q] Re (-1)^(j-i) z(i, j) obs_w[q][(j - i) mod N] / N,
+// i.e. the trace, coordinate averages and, through the signed cyclic diagonal sums, the
+// momentum averages (the weights are DFTs of the momentum functions).
+template <typename C, typename T2>
+__device__ __forceinline__ void cyc_block_partials(const PassArgs &a, const T2 *v, bool active, int line, int u,
+                                                   long long b, long long blk, int nl, int tid) {
+    constexpr int N = C::N, E = C::E, T = C::T;
+    const int nd = a.obs_nd, nw = a.obs_nw, nc = 1 + nd + nw;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (active) {
+        T2 lf = mk<T2>(1.0, 0.0);
+        if (a.obs_lf) lf = reinterpret_cast<const T2 *>(a.obs_lf)[b * a.lf_bstride + line];
+        const T2 *ef = a.obs_ef ? reinterpret_cast<const T2 *>(a.obs_ef) + b * a.ef_bstride + u : nullptr;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        if (q < nc) {   // nc is uniform: only the components in use are reduced
-            double s = acc[q];
+        for (int k = 0; k < E; ++k) {
+            const int e = u + T * k;
+            const T2 z = ef ? cmul(v[k], cmul(lf, ef[T * k])) : cmul(v[k], lf);
+            const double zx = double(z.x), zy = double(z.y);
+            const int d = (line - e) & (N - 1);
+            const double sx = (d & 1) ? -zx : zx, sy = (d & 1) ? -zy : zy;
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-            if (lane == 0) ored[wid][q] = s;
+            for (int q = 0; q < 7; ++q) {
+                if (q < nw) {
+                    const double2 h = a.obs_w[q * N + d];
+                    acc[1 + nd + q] += sx * h.x - sy * h.y;
+                }
+            }
+            if (e == line) {
+                acc[0] += zx;
+#pragma unroll
+                for (int q = 0; q < 7; ++q)
+                    if (q < nd) acc[1 + q] += zx * a.obs_n2w[q * N + line];
+            }
         }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q >= 1 + nd && q < nc) acc[q] *= 1.0 / N;
     }
-    __syncthreads();
-    if (tid < nc) {
-        double s = 0.0;
-        for (int q = 0; q < C::THREADS / 64; ++q) s += ored[q][tid];
-        const long long g0 = blk * LPB;
-        const long long bb = g0 / nl;
-        const long long bi = (g0 - bb * nl) / LPB;
-        a.obs_partials[(bb * (nl / LPB) + bi) * nc + tid] = s;
-    }
+    store_block_partials<C>(acc, nc, a, blk, nl, tid);
 }
 
 // Multiply the thread's elements v[k] (element e = u + T k of its line) by the chirp
@@ -508,6 +565,8 @@ void k_line_pass(PassArgs a) {
     } else if constexpr (C::DA != 0) {
         stockham<T2, N, E, 1, C::DA, LAY, C::TWM, C::XS>(v, L, l_loc, u, tw);
     }
+
+    if constexpr (C::OBS == 3) cyc_block_partials<C>(a, v, active, line, u, b, blk, nl, tid);
 
     if (a.has_factor) {
         // line-constant part: scale * dscale[b] * line_fac[b][line]

@@ -84,7 +84,7 @@ class Plan:
             chirp=0.0, lstep=0.0, lcenter=0.0, estep=0.0, ecenter=0.0, lfft=0, efft=0,
             scale=1.0, dscale=None, out=None, out_mode=0, stream=None, ctab=None, ctab_bstride=0,
             chirp_out=0.0, obs_weights=None, obs_nw=0, obs_partials=None, obs_mode=0, obs_norm2=0,
-            obs_norm2_weights=None, in_scaled_out=None):
+            obs_norm2_weights=None, in_scaled_out=None, obs_line_fac=None, obs_elem_fac=None, obs_diag_nw=0):
         ps = _lib.aw_pass()
         ps.axis, ps.dir_a, ps.dir_b = axis, dir_a, dir_b
         ps.sign_in, ps.sign_out, ps.out_mode = sign_in, sign_out, out_mode
@@ -100,6 +100,7 @@ class Plan:
         ps.obs_weights, ps.obs_nw, ps.obs_partials = _p(obs_weights), obs_nw, _p(obs_partials)
         ps.obs_mode, ps.obs_norm2, ps.obs_norm2_weights = obs_mode, int(obs_norm2), _p(obs_norm2_weights)
         ps.in_scaled_out = _p(in_scaled_out)
+        ps.obs_line_fac, ps.obs_elem_fac, ps.obs_diag_nw = _p(obs_line_fac), _p(obs_elem_fac), int(obs_diag_nw)
         _lib.call("aw_pass_run", self.handle, ctypes.c_void_p(_p(data)), ctypes.byref(ps),
                   ctypes.c_void_p(stream if stream is not None else current_stream()))
 

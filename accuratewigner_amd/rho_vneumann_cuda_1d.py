@@ -16,7 +16,7 @@ import numpy as np
 from . import graphs, rtc
 from ._params import KeywordParams
 from .device import DeviceArray, current_stream, empty, require_gpu, to_device
-from .engine import FWD, INV, Plan, mem_info
+from .engine import FWD, INV, Plan, finish_sums, mem_info
 
 SQRT2 = np.sqrt(2.0)
 CHUNK_STEPS = 256
@@ -25,6 +25,10 @@ CHUNK_STEPS = 256
 class RhoVNeumannCUDA1D(KeywordParams):
     """Second-order split-operator propagator of the von Neumann equation for rho(x, x', t)
     with H = K(p, t) + V(x, t) (reference: rho_vneumann_cuda_1d.py:15-21), on MI355X."""
+
+    # Ehrenfest steps as merged passes whose column pass records the trace and averages of
+    # the closed step (two passes a step); False keeps three passes and a reduction a step
+    merged_ehrenfest = True
 
     def __init__(self, **kwargs):
         """Keywords as rho_vneumann_cuda_1d.py:22-47 (X_gridDIM, X_amplitude, V, K, dt, and
@@ -229,12 +233,13 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
             ts[k] = t
         return ts
 
-    def _chunk_factors(self, ts, merged=True, bufs=None):
+    def _chunk_factors(self, ts, merged=True, bufs=None, both=False):
         """FP64 phase tables (hipRTC) and factor vectors for steps at times ts.  merged: the
         closing half-kick of a step and the opening one of the next share one column pass
         (n + 1 column vectors); otherwise every step has its own opening and closing
         vectors (2 n column vectors, open at index 2 s, close at 2 s + 1).  bufs: write the
-        times, tables and (unmerged) factors into these preallocated arrays."""
+        times, tables and (unmerged) factors into these preallocated arrays; both: also the
+        merged column vectors, into bufs["facm"]."""
         n = len(ts)
         N, dt, p = self.X_gridDIM, self.dt, self.plan
         if bufs is None:
@@ -269,9 +274,19 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
                           out_offset=v, out_vstride=2)
                 p.factors(colE, N, n, A=Vt.ptr, ca=0.5 * dt, mag=bx, alt_sign=True, scale=scale,
                           out_offset=v, out_vstride=2)
+            if not both:
+                return colL, colE, rowL, rowE
+            self._merged_cols(Vt, n, bx, *bufs["facm"])
             return colL, colE, rowL, rowE
         colL = p.new_factors(N, n + 1)
         colE = p.new_factors(N, n + 1)
+        self._merged_cols(Vt, n, bx, colL, colE)
+        return colL, colE, rowL, rowE
+
+    def _merged_cols(self, Vt, n, bx, colL, colE):
+        """Column vectors of n merged steps: 0 opens step 0, s (0 < s < n) closes step s - 1
+        and opens step s, n closes step n - 1 (with the 1 / N of the inverse FFT)."""
+        N, dt, p = self.X_gridDIM, self.dt, self.plan
         p.factors(colL, N, 1, A=Vt.ptr, ca=-0.5 * dt, mag=bx, alt_sign=True)
         p.factors(colE, N, 1, A=Vt.ptr, ca=0.5 * dt, mag=bx, alt_sign=True)
         if n > 1:
@@ -282,7 +297,6 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
         last = Vt.ptr + (n - 1) * N * 8
         p.factors(colL, N, 1, A=last, ca=-0.5 * dt, mag=bx, alt_sign=True, out_offset=n)
         p.factors(colE, N, 1, A=last, ca=0.5 * dt, mag=bx, alt_sign=True, scale=1.0 / N, out_offset=n)
-        return colL, colE, rowL, rowE
 
     def _run_chunk(self, ts, dscale=None):
         """Steps at times ts as 2n+1 fused passes."""
@@ -321,6 +335,31 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
                               self.dX, self._dscale)
             self._scale_pending = True
 
+    def _ehrenfest_merged_chunk(self, n):
+        """n steps as 2 n + 1 passes, [EV_0 -> I0] ([F1 -> EK_s -> I1] [F0 -> EV_s EV_s+1 -> I0])*
+        ending [F0 -> EV_n-1]: each merged column pass also multiplies the values after its
+        first FFT by step s's closing vectors, which is rho_s exactly as an unmerged step ends,
+        and stores the block partials of its trace and Ehrenfest sums (obs_mode 2) in a ring;
+        one launch per chunk adds them into the stage rows.  No per-step normalisation: the
+        steps are linear and the averages are ratios to the recorded trace.  Only launches,
+        from fixed addresses."""
+        bufs, p, rho = self._ehr, self.plan, self.rho
+        N, csz = self.X_gridDIM, self.plan.fsize
+        colL, colE, rowL, rowE = bufs["fac"]
+        mL, mE = bufs["facm"]
+        xw, pwc, ring, nblk = bufs["xw"], bufs["pwc"], bufs["ring"], bufs["nblk"]
+        nc = 7
+        p.run(rho, 0, 0, INV, line_fac=mL, elem_fac=mE)
+        for k in range(n):
+            last = k == n - 1
+            p.run(rho, 1, FWD, INV, line_fac=rowL.ptr + k * N * csz, elem_fac=rowE.ptr + k * N * csz)
+            p.run(rho, 0, FWD, 0 if last else INV, line_fac=mL.ptr + (k + 1) * N * csz,
+                  elem_fac=mE.ptr + (k + 1) * N * csz, obs_mode=2,
+                  obs_line_fac=colL.ptr + (2 * k + 1) * N * csz, obs_elem_fac=colE.ptr + (2 * k + 1) * N * csz,
+                  obs_weights=pwc.ptr + k * 3 * N * 16, obs_nw=3, obs_norm2_weights=xw.ptr + k * 3 * N * 8,
+                  obs_diag_nw=3, obs_partials=ring.ptr + k * nblk * nc * 8)
+        finish_sums(ring, nblk, nc, n, bufs["stage"])
+
     def single_step_propagation(self):
         """rho_vneumann_cuda_1d.py:450-468: unnormalised (no 1/N^2, :464 is commented out)."""
         self._flush_scale()
@@ -344,11 +383,13 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
         self._flush_scale()
         N = self.X_gridDIM
         CH = CHUNK_STEPS
-        ring = empty((steps, 8), np.float64)
         exprs_x = ["X", self.diff_V, self.V]
         exprs_p = [self.diff_K, "P", self.K]
         if self._ehr is None:
             p = self.plan
+            # merged steps when the column pass stores block partials at this size
+            nblk = p.obs_blocks(0) if self.merged_ehrenfest else 0
+            ncol = 8 if nblk == 0 else 7
             # DFTs of the chunk's momentum weights: CH x 3 rows of N points, FP64
             self._cplan = Plan(1, N, CH * len(exprs_p), "fp64")
             self._ehr = dict(ts=empty((CH,), np.float64), Vt=empty((CH, N), np.float64),
@@ -357,13 +398,19 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
                                   p.new_factors(N, CH), p.new_factors(N, CH)),
                              xw=empty((CH, len(exprs_x), N), np.float64),
                              pwc=DeviceArray((CH, len(exprs_p), N), np.complex128),
-                             stage=empty((CH, 8), np.float64), graphs={})
+                             stage=empty((CH, ncol), np.float64), graphs={}, nblk=nblk)
+            if nblk:
+                self._ehr["facm"] = (p.new_factors(N, CH + 1), p.new_factors(N, CH + 1))
+                self._ehr["ring"] = empty((CH, nblk, 7), np.float64)
             self._obs_module(exprs_x)
             self._obs_module(exprs_p)
-            # the observable scratch is allocated by the first call, never in a capture
-            self.plan.cyclic_diag_obs(self.rho, self._ehr["xw"], 3, self._ehr["pwc"], 3, self._ehr["stage"], 8)
+            if not nblk:
+                # the observable scratch is allocated by the first call, never in a capture
+                self.plan.cyclic_diag_obs(self.rho, self._ehr["xw"], 3, self._ehr["pwc"], 3, self._ehr["stage"], 8)
         bufs = self._ehr
-        if not self._scale_pending:
+        merged = bufs["nblk"] > 0
+        ring = empty((steps, bufs["stage"].shape[1]), np.float64)
+        if not merged and not self._scale_pending:
             # every step applies the pending scale: start from 1 so that all chunks are alike
             self._dscale.tensor[:, 0] = 1.0
             self._dscale.tensor[:, 1] = 0.0
@@ -371,14 +418,21 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
         ts_all = to_device(ts)   # one upload; each chunk copies its slice on the device
         for c0 in range(0, steps, CH):
             n = min(CH, steps - c0)
-            self._chunk_factors(ts_all.tensor[c0:c0 + n], merged=False, bufs=bufs)
+            self._chunk_factors(ts_all.tensor[c0:c0 + n], merged=False, bufs=bufs, both=merged)
             # the observable weights of all the chunk's times at once
             self._weights_t(exprs_x, bufs["ts"], n, bufs["xw"])
             self._weights_t(exprs_p, bufs["ts"], n, bufs["pwc"], complex_out=True)
             self._cplan.fft(bufs["pwc"], 1, FWD)
-            graphs.run_chunk(bufs["graphs"], "ehrenfest", lambda n=n: self._ehrenfest_chunk(n), n == CH)
+            if merged:
+                graphs.run_chunk(bufs["graphs"], "merged", lambda n=n: self._ehrenfest_merged_chunk(n), n == CH)
+            else:
+                graphs.run_chunk(bufs["graphs"], "ehrenfest", lambda n=n: self._ehrenfest_chunk(n), n == CH)
             ring.tensor[c0:c0 + n].copy_(bufs["stage"].tensor[:n])
         self.t = ts[-1]
+        if merged:
+            # rho /= Re tr(rho) dX with the last step's recorded trace (:443)
+            self.plan.scale_from_stats(ring.ptr + (steps - 1) * 7 * 8, 7, 0, 0, self.dX, self._dscale)
+            self._scale_pending = True
         self._flush_scale()
         r = ring.get()
         tr = r[:, 0]

@@ -121,6 +121,21 @@ typedef struct aw_pass {
      * division of rho by its norm riding on the first pass of its Wigner transform
      * (normalize_rho_wigner, wigner_moyal_cuda_1d.py:608-622).  NULL: none. */
     void *in_scaled_out;
+    /* obs_mode 2 (column step passes of square items, [forward, factor(, inverse)]): block
+     * partials of the von Neumann observables of the state between two steps,
+     *   z(i, j) = (line j right after the first FFT)[i] * obs_line_fac[b][j] * obs_elem_fac[b][i]
+     * (the closing factor vectors of the step, complex in the plan's precision, with the
+     * batch strides of line_fac / elem_fac; NULL: ones), into obs_partials[b][k][c]:
+     *   c = 0:                 Re sum_j z(j, j)                              (the trace)
+     *   c = 1 .. nd:           Re sum_j z(j, j) obs_norm2_weights[c - 1][j]  (real [nd][n])
+     *   c = 1 + nd .. nd + nw: Re sum_ij (-1)^(j-i) z(i, j) obs_weights[q][(j - i) mod n] / n
+     *                          (complex128 [nw][n]: the DFTs of momentum weights, so that the
+     *                          sum is Re sum_k w(P_k) rho~(k, k) of the momentum representation)
+     * with nd = obs_diag_nw, nw = obs_nw, 1 + nd + nw <= 8 and obs_norm2 = 0.  This is the
+     * trace and the Ehrenfest averages of rho_vneumann_cuda_1d.py:441-448 and :596-644 taken
+     * inside the merged column pass that closes one step and opens the next. */
+    const void *obs_line_fac, *obs_elem_fac;
+    int obs_diag_nw;
 } aw_pass;
 
 int aw_pass_run(aw_plan *plan, void *data, const aw_pass *pass, aw_stream stream);

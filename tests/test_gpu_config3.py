@@ -83,3 +83,35 @@ def test_strong_field_rv_full_mixed_vs_fp64():
         e = np.max(np.abs(obs["mixed"][n] - obs["fp64"][n])) / np.max(np.abs(obs["fp64"][n]))
         print("%s: mixed vs fp64 max error / max |value| %.3e" % (n, e), flush=True)
         assert e < 3e-2, n
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "c64"])
+@pytest.mark.parametrize("N", [64, 256, 1024, 2048])
+def test_rv_merged_ehrenfest_equals_per_step(prec, N):
+    """Merged Ehrenfest steps (two passes a step, the trace and averages of each closed step
+    stored by the merged column pass) against the per-step path (three passes, a reduction
+    of the stored rho and a normalisation every step): the same lists and final rho, over
+    two chunks and a second call, with a time-dependent field and an absorbing edge.  At 64
+    points the column pass has no partials and the per-step path runs in both."""
+    from accuratewigner_amd import RhoVNeumannCUDA1D
+    from accuratewigner_amd.strong_field import rv_sys_params
+    p = rv_sys_params(X_gridDIM=N, X_amplitude=50. * max(N, 256) / 256, dt=0.02, omega=0.6, cycles=1)
+    init = "exp(-pow(X - 1., 2) / 2. + cuda_complex(0., 0.3) * X) * exp(-pow(X_prime - 1., 2) / 2. - cuda_complex(0., 0.3) * X_prime)"
+    out = {}
+    for merged in (True, False):
+        g = RhoVNeumannCUDA1D(precision=prec, **p)
+        g.merged_ehrenfest = merged
+        g.set_rho(init)
+        g.propagate(300)
+        g.propagate(40)
+        assert (g._ehr["nblk"] > 0) == (merged and N > 64)
+        out[merged] = (g.rho.get().astype(np.complex128),
+                       {n: np.array(getattr(g, n)) for n in ("X_average", "P_average", "X_average_RHS",
+                                                              "P_average_RHS", "hamiltonian_average")})
+        del g
+    tol = 1e-11 if prec == "fp64" else 2e-5
+    (r1, l1), (r0, l0) = out[True], out[False]
+    assert np.linalg.norm(r1 - r0) / np.linalg.norm(r0) < tol
+    for name in l0:
+        assert l1[name].shape == (340,)
+        assert np.max(np.abs(l1[name] - l0[name])) < tol * max(1.0, np.max(np.abs(l0[name]))), name
