@@ -181,9 +181,10 @@ static int pass_run_n(aw_plan *p, void *data, const aw_pass *ps, int nbatch, aw_
         if (!ps->obs_partials) return set_error(AW_ERR_INVALID_VALUE, "obs_mode 2 needs an obs_partials buffer");
         if (!strided || p->rows != p->cols)
             return set_error(AW_ERR_INVALID_VALUE, "obs_mode 2 needs a column pass over square items");
-        if (ps->obs_nw < 0 || ps->obs_diag_nw < 0 || 1 + ps->obs_nw + ps->obs_diag_nw > 8)
-            return set_error(AW_ERR_INVALID_VALUE, "obs_mode 2: 1 + obs_nw + obs_diag_nw must be at most 8");
-        if ((ps->obs_nw > 0 && !ps->obs_weights) || (ps->obs_diag_nw > 0 && !ps->obs_norm2_weights))
+        if (ps->obs_nw < 1 || ps->obs_nw > kVnMaxNw || ps->obs_diag_nw < 0 || ps->obs_diag_nw > kVnMaxNd)
+            return set_error(AW_ERR_INVALID_VALUE, "obs_mode 2: obs_nw must be 1..%d and obs_diag_nw 0..%d",
+                             kVnMaxNw, kVnMaxNd);
+        if (!ps->obs_weights || (ps->obs_diag_nw > 0 && !ps->obs_norm2_weights))
             return set_error(AW_ERR_INVALID_VALUE, "obs_mode 2: NULL weights");
         if (ps->obs_norm2) return set_error(AW_ERR_INVALID_VALUE, "obs_mode 2 takes no obs_norm2");
         a.obs_w = static_cast<const double2 *>(ps->obs_weights);

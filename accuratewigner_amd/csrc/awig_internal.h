@@ -47,6 +47,10 @@ struct PassArgs {
     void *aux_out;          // general passes: the loaded input times dscale[b], stored here
 };
 
+// obs_mode 2: at most this many cyclic (momentum) and diagonal (coordinate) weight rows
+constexpr int kVnMaxNw = 3;
+constexpr int kVnMaxNd = 4;
+
 struct FactorArgs {
     void *out; long long out_stride;
     int n, nvec, mode;

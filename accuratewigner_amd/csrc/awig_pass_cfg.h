@@ -84,7 +84,8 @@ template <typename R, int N> constexpr long long small_lines_max() {
 
 // The step kinds of up to 8 elements per thread load their factor vectors and mask with the
 // data (PF = 2), so that their latency overlaps the data loads instead of following the first
-// FFT.
+// FFT; not the kinds with von Neumann observables (OBS 3), whose epilogue after the first FFT
+// would then hold those registers too (FP64 1024-point columns: 256 VGPRs and spills).
 // Columns of 1024 points go 4 to a block (32 / 64-byte row segments): the von Neumann step
 // on one 1024^2 state (three passes) takes 64.3 instead of 68.1 us in FP64 and 45.8 instead
 // of 49.6 us in mixed precision.  At 256 and 512 points, and with 1 or 8 columns at 1024,
@@ -108,7 +109,7 @@ using PassK = KCfg<R, N, PassTune<R, N, STRIDED>::E, PassTune<R, N, STRIDED>::LP
 
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0, typename S = R>
 using PassKS = KCfg<R, N, SmallTune<R, N, STRIDED>::E, SmallTune<R, N, STRIDED>::LPB, STRIDED, 1, DA,
-                    DB, GEN, 1, GEN ? 0 : SmallTune<R, N, STRIDED>::PF, 4, 5, OBS, 0, S>;
+                    DB, GEN, 1, (GEN || OBS == 3) ? 0 : SmallTune<R, N, STRIDED>::PF, 4, 5, OBS, 0, S>;
 
 template <typename C>
 int launch_cfg(const PassArgs &a, hipStream_t st) {

@@ -289,7 +289,17 @@ struct KCfg {
     static constexpr int THREADS = LPB * T;
     using LAY = LdsLayout<STRIDED, N, LPB, PADC_, PADR_>;
     static constexpr int XCH_ELEMS = XS ? (LAY::SIZE + 1) / 2 : LAY::SIZE;   // in complex units
-    static constexpr int LDS_ELEMS = XCH_ELEMS + (TWM != 0 ? N : 0);
+    // OBS 3: elements whose weight gathers are in flight together (register budget)
+    static constexpr int OBS3_KB = E < 4 ? E : (E >= 16 ? 1 : 4);
+    static constexpr int BASE_ELEMS = XCH_ELEMS + (TWM != 0 ? N : 0);
+    // OBS 3: the cyclic weights (kVnMaxNw rows of N complex FP64) and the closing element
+    // vector (N complex) staged in LDS after the exchange and twiddle regions, where they fit
+    static constexpr int OBS3_WOFF = (BASE_ELEMS + 1) & ~1;                 // 16-byte aligned
+    static constexpr int OBS3_ELEMS =
+        kVnMaxNw * N * int(sizeof(double2) / sizeof(typename Cx<R>::T)) + N;
+    static constexpr bool OBS3_LDS =
+        OBS == 3 && (OBS3_WOFF + OBS3_ELEMS) * int(sizeof(typename Cx<R>::T)) <= 160 * 1024;
+    static constexpr int LDS_ELEMS = OBS3_LDS ? OBS3_WOFF + OBS3_ELEMS : BASE_ELEMS;
 };
 
 // Block partials of the weighted sums of a pass's line values v (all lines of the block
@@ -368,50 +378,89 @@ __device__ __forceinline__ void obs_block_partials(const PassArgs &a, const T2 *
     store_block_partials<C>(acc, nc, a, blk, nl, tid);
 }
 
-// Block partials of the von Neumann observables of a square item's state between two steps,
-// from a column pass (line j = column j, element e = row i): the values right after the
-// first FFT times the step's closing vectors, z(i, j) = v * (obs_lf[j] * obs_ef[i]) (the same
-// product as the closing pass of an unmerged step), give
+// The von Neumann observables of a square item's state between two steps, from a column
+// pass (line j = column j, element e = row i): the values right after the first FFT times
+// the step's closing vectors, z(i, j) = v * (obs_lf[j] * obs_ef[i]) (the same product as the
+// closing pass of an unmerged step), give
 //   [0] Re z(j, j), [1 + q] Re z(j, j) obs_n2w[q][j]  (on the diagonal element of the line),
 //   [1 + nd + q] Re (-1)^(j-i) z(i, j) obs_w[q][(j - i) mod N] / N,
 // i.e. the trace, coordinate averages and, through the signed cyclic diagonal sums, the
 // momentum averages (the weights are DFTs of the momentum functions).
+// (The thread's sums; the block sums are stored at the end of the pass, so that the block
+// barrier does not hold the factor registers of the merged pass.)
 template <typename C, typename T2>
-__device__ __forceinline__ void cyc_block_partials(const PassArgs &a, const T2 *v, bool active, int line, int u,
-                                                   long long b, long long blk, int nl, int tid) {
+__device__ __forceinline__ void cyc_partials(const PassArgs &a, const T2 *v, bool active, int line, int u,
+                                             long long b, double (&acc)[8], const T2 *smem) {
     constexpr int N = C::N, E = C::E, T = C::T;
-    const int nd = a.obs_nd, nw = a.obs_nw, nc = 1 + nd + nw;
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    constexpr int NW = kVnMaxNw, ND = kVnMaxNd;
+    const int nd = a.obs_nd, nw = a.obs_nw;
+    double acc0 = 0.0, accx[ND], accp[NW];
+#pragma unroll
+    for (int q = 0; q < ND; ++q) accx[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) accp[q] = 0.0;
     if (active) {
+        // rows past nw read row 0 and count zero: no branch between the gathers, so that
+        // they are all in flight at once
+        // weights and closing element vector from LDS where the pass staged them
+        const double2 *w0 = C::OBS3_LDS ? reinterpret_cast<const double2 *>(smem + C::OBS3_WOFF) : a.obs_w;
+        const double2 *wq[NW];
+        double mq[NW];
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+            wq[q] = w0 + (q < nw ? q : 0) * N;
+            mq[q] = q < nw ? 1.0 : 0.0;
+        }
         T2 lf = mk<T2>(1.0, 0.0);
         if (a.obs_lf) lf = reinterpret_cast<const T2 *>(a.obs_lf)[b * a.lf_bstride + line];
-        const T2 *ef = a.obs_ef ? reinterpret_cast<const T2 *>(a.obs_ef) + b * a.ef_bstride + u : nullptr;
+        const T2 *ef = C::OBS3_LDS ? smem + C::OBS3_WOFF + kVnMaxNw * N * int(sizeof(double2) / sizeof(T2)) + u
+                       : (a.obs_ef ? reinterpret_cast<const T2 *>(a.obs_ef) + b * a.ef_bstride + u : nullptr);
+        // the loads of KB elements at a time in flight (registers: KB (NW + 1) complex)
+        constexpr int KB = C::OBS3_KB;
 #pragma unroll
-        for (int k = 0; k < E; ++k) {
-            const int e = u + T * k;
-            const T2 z = ef ? cmul(v[k], cmul(lf, ef[T * k])) : cmul(v[k], lf);
-            const double zx = double(z.x), zy = double(z.y);
-            const int d = (line - e) & (N - 1);
-            const double sx = (d & 1) ? -zx : zx, sy = (d & 1) ? -zy : zy;
+        for (int k0 = 0; k0 < E; k0 += KB) {
+            // keep the scheduler from hoisting every batch's loads to the top (that takes
+            // (NW + 1) E complex registers at once and spills)
+            __builtin_amdgcn_sched_barrier(0);
+            double2 h[KB][NW];
+            T2 efv[KB];
 #pragma unroll
-            for (int q = 0; q < 7; ++q) {
-                if (q < nw) {
-                    const double2 h = a.obs_w[q * N + d];
-                    acc[1 + nd + q] += sx * h.x - sy * h.y;
+            for (int k = 0; k < KB; ++k) {
+                const int d = (line - (u + T * (k0 + k))) & (N - 1);
+#pragma unroll
+                for (int q = 0; q < NW; ++q) h[k][q] = wq[q][d];
+                efv[k] = ef ? ef[T * (k0 + k)] : mk<T2>(1.0, 0.0);
+            }
+#pragma unroll
+            for (int k = 0; k < KB; ++k) {
+                const int e = u + T * (k0 + k);
+                const T2 z = cmul(v[k0 + k], cmul(lf, efv[k]));
+                const double zx = double(z.x), zy = double(z.y);
+                const int d = (line - e) & (N - 1);
+                const double sx = (d & 1) ? -zx : zx, sy = (d & 1) ? -zy : zy;
+#pragma unroll
+                for (int q = 0; q < NW; ++q) accp[q] += mq[q] * (sx * h[k][q].x - sy * h[k][q].y);
+                if (e == line) {
+                    acc0 += zx;
+#pragma unroll
+                    for (int q = 0; q < ND; ++q)
+                        if (q < nd) accx[q] += zx * a.obs_n2w[q * N + line];
                 }
             }
-            if (e == line) {
-                acc[0] += zx;
-#pragma unroll
-                for (int q = 0; q < 7; ++q)
-                    if (q < nd) acc[1 + q] += zx * a.obs_n2w[q * N + line];
-            }
         }
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (q >= 1 + nd && q < nc) acc[q] *= 1.0 / N;
     }
-    store_block_partials<C>(acc, nc, a, blk, nl, tid);
+    // components [trace, diagonal sums (nd), cyclic sums / N (nw)] at compile-time slots
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        double x = c == 0 ? acc0 : 0.0;
+#pragma unroll
+        for (int q = 0; q < ND; ++q)
+            if (c == 1 + q && q < nd) x = accx[q];
+#pragma unroll
+        for (int q = 0; q < NW; ++q)
+            if (c == 1 + nd + q && q < nw) x = accp[q] * (1.0 / N);
+        acc[c] = x;
+    }
 }
 
 // Multiply the thread's elements v[k] (element e = u + T k of its line) by the chirp
@@ -494,6 +543,25 @@ void k_line_pass(PassArgs a) {
         }
         tw = tl;
     }
+    if constexpr (C::OBS3_LDS) {
+        // the cyclic weights and the closing element vector into LDS (read after the first
+        // FFT, whose barriers order these writes before them)
+        double2 *wl = reinterpret_cast<double2 *>(smem + C::OBS3_WOFF);
+#pragma unroll
+        for (int r = 0; r < kVnMaxNw; ++r) {
+            if (r < a.obs_nw) {
+#pragma unroll
+                for (int q0 = 0; q0 < N; q0 += C::THREADS)
+                    if (q0 + tid < N) wl[r * N + q0 + tid] = a.obs_w[r * N + q0 + tid];
+            }
+        }
+        T2 *el = smem + C::OBS3_WOFF + kVnMaxNw * N * int(sizeof(double2) / sizeof(T2));
+        const long long bi = (blk * LPB) / a.nlines;   // the block's item
+        const T2 *ef = reinterpret_cast<const T2 *>(a.obs_ef) + bi * a.ef_bstride;
+#pragma unroll
+        for (int q0 = 0; q0 < N; q0 += C::THREADS)
+            if (q0 + tid < N) el[q0 + tid] = a.obs_ef ? ef[q0 + tid] : mk<T2>(1.0, 0.0);
+    }
 
     // Addressing: an item is N x nl (columns) or nl x N (rows) elements; offsets inside an
     // item fit 32 bits (<= 4096^2).  Element e = u + T k of the line sits at
@@ -566,7 +634,8 @@ void k_line_pass(PassArgs a) {
         stockham<T2, N, E, 1, C::DA, LAY, C::TWM, C::XS>(v, L, l_loc, u, tw);
     }
 
-    if constexpr (C::OBS == 3) cyc_block_partials<C>(a, v, active, line, u, b, blk, nl, tid);
+    double cacc[C::OBS == 3 ? 8 : 1];
+    if constexpr (C::OBS == 3) cyc_partials<C>(a, v, active, line, u, b, cacc, smem);
 
     if (a.has_factor) {
         // line-constant part: scale * dscale[b] * line_fac[b][line]
@@ -634,6 +703,7 @@ void k_line_pass(PassArgs a) {
         }
     }
     if constexpr (C::OBS == 1) obs_block_partials<C>(a, v, active, off0, kstep, blk, nl, tid, wpre);
+    if constexpr (C::OBS == 3) store_block_partials<C>(cacc, 1 + a.obs_nd + a.obs_nw, a, blk, nl, tid);
     if (!active) return;
     if (!C::GEN || a.out_mode == 0) {
         S2 *dst = reinterpret_cast<S2 *>(a.out) + item + off0;

@@ -131,7 +131,7 @@ typedef struct aw_pass {
      *   c = 1 + nd .. nd + nw: Re sum_ij (-1)^(j-i) z(i, j) obs_weights[q][(j - i) mod n] / n
      *                          (complex128 [nw][n]: the DFTs of momentum weights, so that the
      *                          sum is Re sum_k w(P_k) rho~(k, k) of the momentum representation)
-     * with nd = obs_diag_nw, nw = obs_nw, 1 + nd + nw <= 8 and obs_norm2 = 0.  This is the
+     * with nd = obs_diag_nw (0..4), nw = obs_nw (1..3) and obs_norm2 = 0.  This is the
      * trace and the Ehrenfest averages of rho_vneumann_cuda_1d.py:441-448 and :596-644 taken
      * inside the merged column pass that closes one step and opens the next. */
     const void *obs_line_fac, *obs_elem_fac;
