@@ -348,7 +348,8 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
         colL, colE, rowL, rowE = bufs["fac"]
         mL, mE = bufs["facm"]
         xw, pwc, ring, nblk = bufs["xw"], bufs["pwc"], bufs["ring"], bufs["nblk"]
-        nc = 7
+        nx, np_ = (len(e) for e in bufs["exprs"])
+        nc = 1 + nx + np_
         p.run(rho, 0, 0, INV, line_fac=mL, elem_fac=mE)
         for k in range(n):
             last = k == n - 1
@@ -356,8 +357,8 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
             p.run(rho, 0, FWD, 0 if last else INV, line_fac=mL.ptr + (k + 1) * N * csz,
                   elem_fac=mE.ptr + (k + 1) * N * csz, obs_mode=2,
                   obs_line_fac=colL.ptr + (2 * k + 1) * N * csz, obs_elem_fac=colE.ptr + (2 * k + 1) * N * csz,
-                  obs_weights=pwc.ptr + k * 3 * N * 16, obs_nw=3, obs_norm2_weights=xw.ptr + k * 3 * N * 8,
-                  obs_diag_nw=3, obs_partials=ring.ptr + k * nblk * nc * 8)
+                  obs_weights=pwc.ptr + k * np_ * N * 16, obs_nw=np_, obs_norm2_weights=xw.ptr + k * nx * N * 8,
+                  obs_diag_nw=nx, obs_partials=ring.ptr + k * nblk * nc * 8)
         finish_sums(ring, nblk, nc, n, bufs["stage"])
 
     def single_step_propagation(self):
@@ -383,14 +384,20 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
         self._flush_scale()
         N = self.X_gridDIM
         CH = CHUNK_STEPS
-        exprs_x = ["X", self.diff_V, self.V]
-        exprs_p = [self.diff_K, "P", self.K]
+        wanted_x = ["X", self.diff_V, self.V]
+        wanted_p = [self.diff_K, "P", self.K]
         if self._ehr is None:
             p = self.plan
             # merged steps when the column pass stores block partials at this size
             nblk = p.obs_blocks(0) if self.merged_ehrenfest else 0
-            ncol = 8 if nblk == 0 else 7
-            # DFTs of the chunk's momentum weights: CH x 3 rows of N points, FP64
+            if nblk:
+                # each distinct expression is summed once (the strong-field case has diff_K = "P")
+                exprs_x = list(dict.fromkeys(e.strip() for e in wanted_x))
+                exprs_p = list(dict.fromkeys(e.strip() for e in wanted_p))
+            else:
+                exprs_x, exprs_p = wanted_x, wanted_p
+            ncol = 8 if nblk == 0 else 1 + len(exprs_x) + len(exprs_p)
+            # DFTs of the chunk's momentum weights: CH x (up to 3) rows of N points, FP64
             self._cplan = Plan(1, N, CH * len(exprs_p), "fp64")
             self._ehr = dict(ts=empty((CH,), np.float64), Vt=empty((CH, N), np.float64),
                              Kt=empty((CH, N), np.float64),
@@ -398,10 +405,11 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
                                   p.new_factors(N, CH), p.new_factors(N, CH)),
                              xw=empty((CH, len(exprs_x), N), np.float64),
                              pwc=DeviceArray((CH, len(exprs_p), N), np.complex128),
-                             stage=empty((CH, ncol), np.float64), graphs={}, nblk=nblk)
+                             stage=empty((CH, ncol), np.float64), graphs={}, nblk=nblk,
+                             exprs=(exprs_x, exprs_p))
             if nblk:
                 self._ehr["facm"] = (p.new_factors(N, CH + 1), p.new_factors(N, CH + 1))
-                self._ehr["ring"] = empty((CH, nblk, 7), np.float64)
+                self._ehr["ring"] = empty((CH, nblk, ncol), np.float64)
             self._obs_module(exprs_x)
             self._obs_module(exprs_p)
             if not nblk:
@@ -409,7 +417,9 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
                 self.plan.cyclic_diag_obs(self.rho, self._ehr["xw"], 3, self._ehr["pwc"], 3, self._ehr["stage"], 8)
         bufs = self._ehr
         merged = bufs["nblk"] > 0
-        ring = empty((steps, bufs["stage"].shape[1]), np.float64)
+        exprs_x, exprs_p = bufs["exprs"]
+        ncol = bufs["stage"].shape[1]
+        ring = empty((steps, ncol), np.float64)
         if not merged and not self._scale_pending:
             # every step applies the pending scale: start from 1 so that all chunks are alike
             self._dscale.tensor[:, 0] = 1.0
@@ -431,16 +441,19 @@ extern "C" __global__ void aw_init(%(st)s *rho, long long n_item)
         self.t = ts[-1]
         if merged:
             # rho /= Re tr(rho) dX with the last step's recorded trace (:443)
-            self.plan.scale_from_stats(ring.ptr + (steps - 1) * 7 * 8, 7, 0, 0, self.dX, self._dscale)
+            self.plan.scale_from_stats(ring.ptr + (steps - 1) * ncol * 8, ncol, 0, 0, self.dX, self._dscale)
             self._scale_pending = True
         self._flush_scale()
         r = ring.get()
         tr = r[:, 0]
-        self.X_average.extend((r[:, 1] / tr).tolist())
-        self.P_average_RHS.extend((-r[:, 2] / tr).tolist())
-        self.X_average_RHS.extend((r[:, 4] / tr).tolist())
-        self.P_average.extend((r[:, 5] / tr).tolist())
-        self.hamiltonian_average.extend(((r[:, 6] + r[:, 3]) / tr).tolist())
+        # stage columns: trace, then the coordinate and the momentum sums of exprs_x / exprs_p
+        cx = [1 + exprs_x.index(e.strip() if merged else e) for e in wanted_x]
+        cp = [1 + len(exprs_x) + exprs_p.index(e.strip() if merged else e) for e in wanted_p]
+        self.X_average.extend((r[:, cx[0]] / tr).tolist())
+        self.P_average_RHS.extend((-r[:, cx[1]] / tr).tolist())
+        self.X_average_RHS.extend((r[:, cp[0]] / tr).tolist())
+        self.P_average.extend((r[:, cp[1]] / tr).tolist())
+        self.hamiltonian_average.extend(((r[:, cp[2]] + r[:, cx[2]]) / tr).tolist())
         return self
 
     # ------------------------------------------------------------------------------------
