@@ -195,6 +195,19 @@ static int pass_run_n(aw_plan *p, void *data, const aw_pass *ps, int nbatch, aw_
         a.obs_nd = ps->obs_diag_nw;
         a.obs_lf = ps->obs_line_fac;
         a.obs_ef = ps->obs_elem_fac;
+    } else if (ps->obs_mode == 3) {
+        // dense weighted sums of the state between two steps, from a column pass
+        if (!ps->obs_partials || !ps->obs_weights)
+            return set_error(AW_ERR_INVALID_VALUE, "obs_mode 3 needs weights and an obs_partials buffer");
+        if (!strided) return set_error(AW_ERR_INVALID_VALUE, "obs_mode 3 needs a column pass");
+        if (ps->obs_nw < 1 || ps->obs_nw > 8) return set_error(AW_ERR_INVALID_VALUE, "obs_mode 3: obs_nw must be 1..8");
+        if (ps->obs_norm2) return set_error(AW_ERR_INVALID_VALUE, "obs_mode 3 takes no obs_norm2");
+        a.obs_w = static_cast<const double2 *>(ps->obs_weights);
+        a.obs_nw = ps->obs_nw;
+        a.obs_partials = ps->obs_partials;
+        a.obs_mode = 3;
+        a.obs_lf = ps->obs_line_fac;
+        a.obs_ef = ps->obs_elem_fac;
     } else if (ps->obs_weights) {
         if (ps->obs_nw < 1 || ps->obs_nw > 8 || !ps->obs_partials)
             return set_error(AW_ERR_INVALID_VALUE, "obs_nw must be 1..8 with an obs_partials buffer");

@@ -69,11 +69,11 @@ int launch_pass(int prec, int n, bool strided, const PassArgs &a, hipStream_t st
         if (a.dir_a == AW_FORWARD && a.dir_b == AW_INVERSE) kind = 7;
         else if (a.dir_a == AW_INVERSE && a.dir_b == 0) kind = 8;
     }
-    if (a.obs_mode == 2) {
+    if (a.obs_mode == 2 || a.obs_mode == 3) {
         if (!strided || (kind != 2 && kind != 3))
             return set_error(AW_ERR_INVALID_VALUE,
-                             "von Neumann observables need a plain [forward, factor(, inverse)] column pass");
-        kind = kind == 2 ? 9 : 10;
+                             "sums of the closed step need a plain [forward, factor(, inverse)] column pass");
+        kind = a.obs_mode == 2 ? (kind == 2 ? 9 : 10) : (kind == 2 ? 11 : 12);
     } else if (a.obs_w) {
         if (a.obs_mode == 0) {
             if (kind != 2 || strided)

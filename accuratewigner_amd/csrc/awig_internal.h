@@ -41,7 +41,8 @@ struct PassArgs {
     int obs_mode;           // 0: partials of the pass output (row passes); 1: of the values
                             // right after the factor multiply (column passes); 2: von Neumann
                             // observables of the state after the first FFT and the closing
-                            // vectors obs_lf / obs_ef (column passes of square items)
+                            // vectors obs_lf / obs_ef (column passes of square items); 3: dense
+                            // weighted sums of that state
     const void *obs_lf, *obs_ef;
     int obs_nd;             // obs_mode 2: diagonal weights (obs_n2w rows)
     void *aux_out;          // general passes: the loaded input times dscale[b], stored here

@@ -109,7 +109,7 @@ using PassK = KCfg<R, N, PassTune<R, N, STRIDED>::E, PassTune<R, N, STRIDED>::LP
 
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0, typename S = R>
 using PassKS = KCfg<R, N, SmallTune<R, N, STRIDED>::E, SmallTune<R, N, STRIDED>::LPB, STRIDED, 1, DA,
-                    DB, GEN, 1, (GEN || OBS == 3) ? 0 : SmallTune<R, N, STRIDED>::PF, 4, 5, OBS, 0, S>;
+                    DB, GEN, 1, (GEN || OBS >= 3) ? 0 : SmallTune<R, N, STRIDED>::PF, 4, 5, OBS, 0, S>;
 
 template <typename C>
 int launch_cfg(const PassArgs &a, hipStream_t st) {
@@ -132,6 +132,7 @@ int launch_cfg(const PassArgs &a, hipStream_t st) {
 //       9 / 10 column step [forward, factor, inverse] / [forward, factor] that stores the
 //         block partials of the von Neumann observables of the values after the first FFT
 //         times the closing vectors of a step (the state between two steps);
+//       11 / 12 the same with dense weighted sums of that state instead;
 //       7 / 8 general pass with the directions [forward, inverse] / [inverse, none] fixed (the
 //         passes of the Wigner transform: one FFT body per slot instead of both directions,
 //         104 -> 92 us for an FP32 1024-point row pass over 24 1024^2 states); the latency
@@ -160,6 +161,13 @@ int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
                     return launch_cfg<PassKS<R, N, STRIDED, -1, 0, 0, 3, S>>(a, st);
                 }
                 return set_error(AW_ERR_INVALID_VALUE, "von Neumann observables: column passes of whole waves only");
+            case 11:
+            case 12:
+                if constexpr (STRIDED && PassKS<R, N, STRIDED, -1, 1, 0>::THREADS % 64 == 0) {
+                    if (kind == 11) return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0, 4, S>>(a, st);
+                    return launch_cfg<PassKS<R, N, STRIDED, -1, 0, 0, 4, S>>(a, st);
+                }
+                return set_error(AW_ERR_INVALID_VALUE, "sums of the closed step: column passes of whole waves only");
             case 7:
             case 8:
             case 1: if (kind == 1) return launch_cfg<PassKS<R, N, STRIDED, 0, 1, 0, 0, S>>(a, st);
@@ -188,6 +196,13 @@ int launch_kind_n(int kind, const PassArgs &a, hipStream_t st) {
             return launch_cfg<PassK<R, N, STRIDED, -1, 0, 0, 3, S>>(a, st);
         }
         return set_error(AW_ERR_INVALID_VALUE, "von Neumann observables: column passes of whole waves only");
+    case 11:
+    case 12:
+        if constexpr (STRIDED && PassK<R, N, STRIDED, -1, 1, 0>::THREADS % 64 == 0) {
+            if (kind == 11) return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0, 4, S>>(a, st);
+            return launch_cfg<PassK<R, N, STRIDED, -1, 0, 0, 4, S>>(a, st);
+        }
+        return set_error(AW_ERR_INVALID_VALUE, "sums of the closed step: column passes of whole waves only");
     case 7: return launch_cfg<PassK<R, N, STRIDED, -1, 1, 1, 0, S>>(a, st);
     case 8: return launch_cfg<PassK<R, N, STRIDED, 1, 0, 1, 0, S>>(a, st);
     case 1: return launch_cfg<PassK<R, N, STRIDED, 0, 1, 0, 0, S>>(a, st);

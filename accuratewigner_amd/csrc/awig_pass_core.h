@@ -279,7 +279,8 @@ struct KCfg {
     static constexpr int N = N_, E = E_, LPB = LPB_, TWM = TWM_, DA = DA_, DB = DB_, GEN = GEN_;
     static constexpr int OBS = OBS_;     // weighted-sum block partials of the output (1) or of the
                                          // values right after the factor multiply (2); von Neumann
-                                         // observables of the closed step (3, columns)
+                                         // observables (3) or dense weighted sums (4) of the closed
+                                         // step (columns)
     static constexpr int XS = XS_;       // 1: LDS exchange in real and imaginary halves
     static constexpr int MINW = MINW_;   // minimum waves per SIMD the register budget must allow
     static constexpr int PF = PF_;       // 1: load the factor vectors with the data, before the first FFT;
@@ -463,6 +464,44 @@ __device__ __forceinline__ void cyc_partials(const PassArgs &a, const T2 *v, boo
     }
 }
 
+// Weighted sums of the state between two steps, from a column pass (obs_mode 3): z = the
+// values right after the first FFT times the step's closing vectors (as above), and
+// Re sum conj(w_q) z over the item for dense complex weights w [nw][rows][cols] shared by the
+// items.  The weight rows are taken one at a time, the E loads of a row issued together.
+template <typename C, typename T2>
+__device__ __forceinline__ void closed_partials(const PassArgs &a, const T2 *v, bool active, int line, int u,
+                                                long long b, int off0, int kstep, int nl, double (&acc)[8]) {
+    constexpr int N = C::N, E = C::E, T = C::T;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = 0.0;
+    if (!active) return;
+    T2 lf = mk<T2>(1.0, 0.0);
+    if (a.obs_lf) lf = reinterpret_cast<const T2 *>(a.obs_lf)[b * a.lf_bstride + line];
+    T2 z[E];
+    if (a.obs_ef) {
+        const T2 *ef = reinterpret_cast<const T2 *>(a.obs_ef) + b * a.ef_bstride + u;
+#pragma unroll
+        for (int k = 0; k < E; ++k) z[k] = cmul(v[k], cmul(lf, ef[T * k]));
+    } else {
+#pragma unroll
+        for (int k = 0; k < E; ++k) z[k] = cmul(v[k], lf);
+    }
+    const long long nit = (long long)N * nl;
+    const double2 *w = a.obs_w + off0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        if (q < a.obs_nw) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const double2 c = w[q * nit + (long long)k * kstep];
+                s += c.x * double(z[k].x) + c.y * double(z[k].y);
+            }
+            acc[q] = s;
+        }
+    }
+}
+
 // Multiply the thread's elements v[k] (element e = u + T k of its line) by the chirp
 // exp(i c lcoord ecoord(e)), ecoord(e) = estep (q - ecenter) with q = e, or its FFT-ordered
 // index (e + N/2) % N.  The phase is linear in k between the wrap of the FFT order, which
@@ -634,8 +673,9 @@ void k_line_pass(PassArgs a) {
         stockham<T2, N, E, 1, C::DA, LAY, C::TWM, C::XS>(v, L, l_loc, u, tw);
     }
 
-    double cacc[C::OBS == 3 ? 8 : 1];
+    double cacc[(C::OBS == 3 || C::OBS == 4) ? 8 : 1];
     if constexpr (C::OBS == 3) cyc_partials<C>(a, v, active, line, u, b, cacc, smem);
+    if constexpr (C::OBS == 4) closed_partials<C>(a, v, active, line, u, b, off0, kstep, nl, cacc);
 
     if (a.has_factor) {
         // line-constant part: scale * dscale[b] * line_fac[b][line]
@@ -704,6 +744,7 @@ void k_line_pass(PassArgs a) {
     }
     if constexpr (C::OBS == 1) obs_block_partials<C>(a, v, active, off0, kstep, blk, nl, tid, wpre);
     if constexpr (C::OBS == 3) store_block_partials<C>(cacc, 1 + a.obs_nd + a.obs_nw, a, blk, nl, tid);
+    if constexpr (C::OBS == 4) store_block_partials<C>(cacc, a.obs_nw, a, blk, nl, tid);
     if (!active) return;
     if (!C::GEN || a.out_mode == 0) {
         S2 *dst = reinterpret_cast<S2 *>(a.out) + item + off0;

@@ -280,6 +280,7 @@ extern "C" __global__ void aw_eval_mask(int which, const double *xj, int nj, con
             body = "".join("        acc[%d] += w * (%s);\n" % (k, e) for k, e in enumerate(exprs))
             grid_body = "".join("        out[%d * (long long)n_item + idx] = make_double2((%s), 0.);\n" % (k, e)
                                 for k, e in enumerate(exprs))
+            pt_body = "".join("        o[%d * (long long)npts] = (%s);\n" % (k, e) for k, e in enumerate(exprs))
             src = (rtc.PRELUDE + self.cuda_consts
                    + "\n__device__ double K(double P, double t)\n{\n    return (%s);\n}\n" % self.K
                    + "\n__device__ double V(double X, double t)\n{\n    return (%s);\n}\n" % self.V
@@ -315,7 +316,22 @@ extern "C" __global__ void aw_obs_grid(double2 *out, int n_item, double t)
         const double P = dP * (i - 0.5 * P_gridDIM);
 %(grid_body)s    }
 }
-""" % dict(st=self._state_t(), grid_body=grid_body))
+// the expressions at npts grid points (flat indices i * X_gridDIM + j) and nt times:
+// out[s][k][q] = f_k(X, P, ts[s]) at point pts[q]
+extern "C" __global__ void aw_obs_points(const int *pts, int npts, const double *ts, int nt, double *out)
+{
+    const long long total = (long long)nt * npts;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int s = (int)(idx / npts), q = (int)(idx %% npts);
+        const int g = pts[q], i = g / X_gridDIM, j = g - i * X_gridDIM;
+        const double X = dX * (j - 0.5 * X_gridDIM);
+        const double P = dP * (i - 0.5 * P_gridDIM);
+        const double t = ts[s];
+        double *o = out + (long long)s * %(nobs)d * npts + q;
+%(pt_body)s    }
+}
+""" % dict(st=self._state_t(), grid_body=grid_body, nobs=nobs, pt_body=pt_body))
             mod = rtc.Module(src, "wigner_moyal_observables.hip")
             self._observable_kernels[key] = mod
         return mod
@@ -371,6 +387,92 @@ extern "C" __global__ void aw_obs_grid(double2 *out, int n_item, double t)
         self._obs_plan = Plan(Np, Nx, len(exprs), "fp64")   # lives with the system (its launches are queued)
         wm_ops.rho2wigner_adjoint(self._obs_plan, self._grid, g)
         return g
+
+    def _obs_points(self, exprs, pts, ts):
+        """f_k(X, P, t) at the flat grid indices pts for each time in ts: host [nt, nexprs, npts]."""
+        mod = self._observable_kernel(exprs)
+        pts_d = to_device(np.ascontiguousarray(pts, dtype=np.int32))
+        ts_d = to_device(np.ascontiguousarray(ts, dtype=np.float64))
+        out = empty((len(ts), len(exprs), len(pts)), np.float64)
+        blocks, threads = rtc.launch_cfg(len(ts) * len(pts))
+        mod.launch("aw_obs_points", (blocks,), (threads,),
+                   [pts_d.ptr, ctypes.c_int(len(pts)), ts_d.ptr, ctypes.c_int(len(ts)), out.ptr], current_stream())
+        return out.get()
+
+    # sample times and tolerances of the time-separable observables
+    _TSEP_TAUS = (0.3711, 1.9137, 3.1303, 7.7719, 12.4261, 29.3333)
+    _TSEP_TOL = 1e-9
+
+    def _tsep_setup(self, exprs):
+        """Observables f_e(X, P, t) that depend on t only through a few functions of t, e.g.
+        diff_V = X / (X^2 + a)^1.5 + E(t) or K + V with V = V0(X) + X E(t): at every t they lie
+        in the span of r <= 8 time-independent grid functions, here actual evaluations
+        b_m = f_e(., tau_m) at sample times.  Then sum W f_e(t) = sum_m c_em(t) Re <L^H b_m, rho>:
+        the r weight arrays L^H b_m are formed once, and c(t) comes from the values of f_e(t)
+        at r pivot grid points.  The span is found by a pivoted QR of the expressions sampled
+        at 6 times on 4096 grid points and verified on 16384 more; each call checks it again
+        at its own step times (``_tsep_coeffs``).  None when the observables do not fit."""
+        import scipy.linalg as sla
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        ng = Nx * Np
+        rng = np.random.default_rng(20231)
+        nsamp = min(ng, 4096 + 16384)
+        allp = rng.choice(ng, size=nsamp, replace=False).astype(np.int32)
+        ns1 = min(4096, nsamp // 2)
+        taus = float(self.t) + np.array(self._TSEP_TAUS)
+        vals = self._obs_points(exprs, allp, taus)            # [ntau, ne, nsamp]
+        ti = [time_independent(e, self.cuda_consts) for e in exprs]
+        cols, keys = [], []
+        for k in range(len(exprs)):
+            for j in ([0] if ti[k] else range(len(taus))):
+                cols.append(vals[j, k]); keys.append((k, j))
+        M = np.stack(cols, axis=1)                             # [nsamp, ncols]
+        scale = np.maximum(np.max(np.abs(M), axis=0), 1e-300)
+        _, R, piv = sla.qr(M[:ns1] / scale, mode="economic", pivoting=True)
+        d = np.abs(np.diag(R))
+        r = int(np.sum(d > 1e-10 * d[0])) if d[0] > 0 else 0
+        if r == 0 or r > 8:
+            return None
+        bcols = list(piv[:r])
+        Mb = M[:, bcols]
+        _, _, prow = sla.qr(Mb[:ns1].T, mode="economic", pivoting=True)
+        prow = np.asarray(prow[:r])
+        Bp = Mb[prow]
+        C = np.linalg.solve(Bp, M[prow])
+        err = np.max(np.abs(Mb @ C - M), axis=0) / scale
+        if not np.all(err < self._TSEP_TOL):
+            return None
+        chk = np.setdiff1d(np.arange(ns1, nsamp), prow)[:64]
+        # the basis functions on the whole grid, then their adjoint Wigner transforms
+        G = empty((r, Np, Nx), np.complex128)
+        mod = self._observable_kernel(exprs)
+        blocks, threads = rtc.launch_cfg(ng)
+        tmp = empty((len(exprs), Np, Nx), np.complex128)
+        for m, c in enumerate(bcols):
+            k, j = keys[c]
+            mod.launch("aw_obs_grid", (blocks,), (threads,), [tmp.ptr, ctypes.c_int(ng), ctypes.c_double(taus[j])],
+                       current_stream())
+            G.tensor[m].copy_(tmp.tensor[k])
+        del tmp
+        self._tsep_plan = Plan(Np, Nx, r, "fp64")             # lives with the system (queued launches)
+        wm_ops.rho2wigner_adjoint(self._tsep_plan, self._grid, G)
+        return dict(nb=r, G=G, pts=np.concatenate([allp[prow], allp[chk]]).astype(np.int32),
+                    Bp=Bp, Bchk=Mb[chk], tol=self._TSEP_TOL)
+
+    def _tsep_coeffs(self, exprs, ts):
+        """c[s, e, m] with f_e(., ts[s]) = sum_m c[s, e, m] b_m, from the pivot points; None
+        when a step's values at the check points do not follow (the span does not hold)."""
+        info = self._ehr["tsep"]
+        r = info["nb"]
+        v = self._obs_points(exprs, info["pts"], ts)           # [nt, ne, r + nchk]
+        nt, ne = v.shape[0], v.shape[1]
+        c = np.linalg.solve(info["Bp"], v[:, :, :r].reshape(nt * ne, r).T)     # [r, nt ne]
+        rec = (info["Bchk"] @ c).T.reshape(nt, ne, -1)
+        chk = v[:, :, r:]
+        scale = np.maximum(np.max(np.abs(chk), axis=2, keepdims=True), 1e-300)
+        if not np.all(np.abs(rec - chk) / scale < info["tol"]):
+            return None
+        return c.T.reshape(nt, ne, r)
 
     def _per_item(self, values):
         values = np.asarray(values, dtype=np.float64)
@@ -574,17 +676,18 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         return (not self._tdep_masks() and time_independent(self.V, self.cuda_consts)
                 and time_independent(self.K, self.cuda_consts))
 
-    def _build_chunk_factors(self, ts, merged=True, bufs=None):
+    def _build_chunk_factors(self, ts, merged=True, bufs=None, both=False):
         """The factor vectors of one chunk (``_build_factors``); with a time-independent H and
         persistent buffers, the constant vectors of any chunk, built on first use: merged
         open / mid / close column vectors 0, 1, 2 and row vector 0, or one unmerged step's open
-        and close column vectors 0, 1 and row vector 0."""
+        and close column vectors 0, 1 and row vector 0 (with ``both``, merged into bufs["fac"]
+        and unmerged into bufs["facu"])."""
         if not (self._static and bufs is not None):
-            return self._build_factors(ts, merged, bufs)
-        key = "merged" if merged else "split"
+            return self._build_factors(ts, merged, bufs, both)
+        key = ("merged" if merged else "split") + ("+split" if both else "")
         if bufs.get("static") != key:
             n = 2 if merged else 1
-            self._build_factors(np.full(n, float(self.t) + self.dt), merged, bufs)
+            self._build_factors(np.full(n, float(self.t) + self.dt), merged, bufs, both)
             bufs["static"] = key
         if ts is not None:
             self._stage_times(bufs, ts)    # the step times, for observables that read t
@@ -599,12 +702,13 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             ts_dev.tensor[:n].copy_(ts)   # a device slice: ordered on the stream, no host wait
         return ts_dev
 
-    def _build_factors(self, ts, merged=True, bufs=None):
+    def _build_factors(self, ts, merged=True, bufs=None, both=False):
         """Phase tables (FP64, hipRTC) and the per-pass complex factor vectors of one chunk.
         merged: the closing half-kick of a step and the opening one of the next share one
         column pass (n + 1 column vectors); otherwise each step has its own opening and
         closing vectors (2 n column vectors, open at 2 s, close at 2 s + 1).  bufs: write
-        the times, tables and factors into these preallocated arrays (``_chunk_bufs``)."""
+        the times, tables and factors into these preallocated arrays (``_chunk_bufs``);
+        both: also the unmerged vectors, into bufs["facu"]."""
         n = len(ts)
         Nx, Np, dt = self.X_gridDIM, self.P_gridDIM, self.dt
         if bufs is None:
@@ -623,6 +727,8 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         mask_t = None
         if self._tdep_masks():
             mask_t = self._mask_tables(ts_dev, n, None if bufs is None else bufs.setdefault("mask_t", {}))
+        if both:
+            self._factors_from_tables(n, Vj.ptr, Vi.ptr, Kj.ptr, Ki.ptr, dt, False, dst=bufs["facu"], mask_t=mask_t)
         return self._factors_from_tables(n, Vj.ptr, Vi.ptr, Kj.ptr, Ki.ptr, dt, merged, dst=dst, mask_t=mask_t)
 
     def _chunk_bufs(self, n):
@@ -833,12 +939,29 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             # time-independent observables imply a time-independent V (the Hamiltonian is one of
             # them): the steps are merged and the observables read from each row-pass output
             self._ehr["weights"] = self._closing_adjoint(g) if g is not None else None
+            self._ehr["tsep"] = None
             if g is not None:
                 # block partials of every step of a chunk, added by one launch per chunk
                 nblk = self.plan.row_obs_blocks()
                 self._ehr["wring"] = empty((CH, self.batch, nblk, nobs), np.float64)
                 self._ehr["wnblk"] = nblk
+            elif self.obs_by_adjoint and self._mask_args(0, 1)[2] is None and self.plan.obs_blocks(0) > 0:
+                # observables that depend on t through a few functions of t: merged steps whose
+                # column pass records the closed state's sums against time-independent weights
+                tsep = self._tsep_setup(exprs)
+                if tsep is not None:
+                    nblk = self.plan.obs_blocks(0)
+                    r = tsep["nb"]
+                    self._ehr.update(tsep=tsep, tring=empty((CH, self.batch, nblk, r), np.float64), tnblk=nblk,
+                                     tstage=empty((CH, self.batch, r), np.float64),
+                                     facu=(self.plan.new_factors(self.X_gridDIM, 2 * CH),
+                                           self.plan.new_factors(self.P_gridDIM, 2 * CH),
+                                           self.plan.new_factors(self.P_gridDIM, CH),
+                                           self.plan.new_factors(self.X_gridDIM, CH)))
         bufs = self._ehr
+        coef = self._tsep_coeffs(exprs, ts) if bufs["tsep"] is not None else None
+        if coef is not None:
+            return self._propagate_tsep(exprs, col, steps, ts, ts_all, coef)
         merged = bufs["weights"] is not None
         if merged:
             # the steps are linear, so the reference's per-step normalisation (:584-622) is
@@ -870,6 +993,61 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             vals = sgn * r[:, :, col[k]] / s0
             lst.extend(vals[:, 0].tolist() if self.batch == 1 else list(vals))
         return self.wignerfunction
+
+    def _propagate_tsep(self, exprs, col, steps, ts, ts_all, coef):
+        """The Ehrenfest loop with time-separable observables (``_tsep_setup``): merged steps,
+        each merged column pass storing the closed state's sums against the basis weights; the
+        averages are formed on the host from the coefficients coef[s, e, m]; rho and W are
+        normalised once at the end (the steps are linear, the averages ratios)."""
+        bufs, CH = self._ehr, CHUNK_STEPS
+        r = bufs["tsep"]["nb"]
+        ring = empty((steps, self.batch, r), np.float64)
+        self._flush_scale()
+        for c0 in range(0, steps, CH):
+            n = min(CH, steps - c0)
+            self._build_chunk_factors(ts_all.tensor[c0:c0 + n], merged=True, bufs=bufs, both=True)
+            graphs.run_chunk(bufs["graphs"], ("tsep",) + tuple(exprs), lambda n=n: self._ehrenfest_tsep_chunk(n),
+                             n == CH)
+            ring.tensor[c0:c0 + n].copy_(bufs["tstage"].tensor[:n])
+        self.t = ts[-1]
+        self.normalize_rho_wigner()
+        S = ring.get()                                           # [steps, batch, r]
+        vals = np.einsum("sem,sbm->sbe", coef, S)                # [steps, batch, nexprs]
+        s0 = vals[:, :, 0]
+        for lst, k, sgn in ((self.X_average, 1, 1.), (self.X_average_RHS, 2, 1.),
+                            (self.P_average, 3, 1.), (self.P_average_RHS, 4, -1.),
+                            (self.hamiltonian_average, 5, 1.)):
+            v = sgn * vals[:, :, col[k]] / s0
+            lst.extend(v[:, 0].tolist() if self.batch == 1 else list(v))
+        return self.wignerfunction
+
+    def _ehrenfest_tsep_chunk(self, n):
+        """n merged steps, 2 n + 1 passes, [EV_0 -> I0] ([F1 -> EK_s -> I1] [F0 -> EV_s EV_s+1 -> I0])*
+        ending [F0 -> EV_n-1]: each merged column pass also multiplies its values after F0 by
+        step s's closing vectors (rho_s exactly as an unmerged closing pass stores it) and
+        stores the block partials of Re <L^H b_m, rho_s> (obs_mode 3) in a ring; one launch per
+        chunk adds them.  Only launches, from fixed addresses."""
+        bufs, p, rho = self._ehr, self.plan, self.rho
+        colL, colE, rowL, rowE = bufs["fac"]
+        uL, uE = bufs["facu"][0], bufs["facu"][1]
+        _, _, tabp = self._mask_args(1, 1)
+        csz = p.fsize
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        G, r = bufs["tsep"]["G"], bufs["tsep"]["nb"]
+        ring, nblk = bufs["tring"], bufs["tnblk"]
+        ring_row = self.batch * nblk * r * 8
+        p.run(rho, 0, 0, INV, line_fac=colL, elem_fac=colE)
+        for k in range(n):
+            last = k == n - 1
+            # vectors of step k: row k, merged column k + 1, unmerged closing 2 k + 1 (row 0,
+            # merged 1 or 2 and closing 1 with a time-independent H)
+            kr, kc, ku = (0, 2 if last else 1, 1) if self._static else (k, k + 1, 2 * k + 1)
+            p.run(rho, 1, FWD, INV, line_fac=rowL.ptr + kr * Np * csz, elem_fac=rowE.ptr + kr * Nx * csz, mask=tabp)
+            p.run(rho, 0, FWD, 0 if last else INV, line_fac=colL.ptr + kc * Nx * csz,
+                  elem_fac=colE.ptr + kc * Np * csz, obs_mode=3, obs_weights=G, obs_nw=r,
+                  obs_partials=ring.ptr + k * ring_row, obs_line_fac=uL.ptr + ku * Nx * csz,
+                  obs_elem_fac=uE.ptr + ku * Np * csz)
+        finish_sums(ring, nblk, r, n * self.batch, bufs["tstage"])
 
     def _merged_chunk(self, n, b0=0, nb=0):
         """n merged steps (2 n + 1 passes) from the chunk buffers, over the items b0 .. b0 + nb

@@ -134,6 +134,11 @@ typedef struct aw_pass {
      * with nd = obs_diag_nw (0..4), nw = obs_nw (1..3) and obs_norm2 = 0.  This is the
      * trace and the Ehrenfest averages of rho_vneumann_cuda_1d.py:441-448 and :596-644 taken
      * inside the merged column pass that closes one step and opens the next. */
+    /* obs_mode 3 (column step passes, [forward, factor(, inverse)]): the weighted sums
+     * Re sum conj(obs_weights[q]) z (q < obs_nw <= 8, complex128 [obs_nw][rows][cols] shared by
+     * the items, obs_norm2 = 0) of the same state z between two steps, into obs_partials[b][k][q]:
+     * the Wigner-Moyal Ehrenfest averages of a step (wigner_moyal_cuda_1d.py:584-606) as
+     * Re <L^H f, rho> when the phases change every step (a time-dependent V). */
     const void *obs_line_fac, *obs_elem_fac;
     int obs_diag_nw;
 } aw_pass;

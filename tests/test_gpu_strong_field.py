@@ -25,6 +25,9 @@ def test_strong_field_frames_match_oracle(tmp_path):
         run.run(max_frames=3)
         run.save_ehrenfest()
     qs = run.quant_sys
+    # the laser makes V and two of the observables time-dependent: they are read from merged
+    # steps against time-independent weights
+    assert qs._ehr["tsep"] is not None
     F, om, dt = p["F"], p["omega"], p["dt"]
     E = lambda t: -F * np.sin(om * t) * np.sin(om * t / 16.) ** 2    # noqa: E731
     # the mask sees XX_max, XX_prime_max as #define'd values

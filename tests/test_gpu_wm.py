@@ -304,3 +304,55 @@ def test_wm_static_factors_equal_per_step_factors(ehrenfest, prec):
     for a, b in zip(Ls, Ld):
         assert np.array_equal(a, b)
 
+
+
+@pytest.mark.parametrize("prec,batch", [("fp64", 1), ("mixed", 1), ("fp64", 2)])
+def test_wm_time_separable_observables_equal_per_step_transform(prec, batch):
+    """Ehrenfest lists with a time-dependent V (a driving field): observables that depend on t
+    through a few functions of t (diff_V = V0'(X) + E(t), K + V = K + V0 + X E(t)) are read
+    from merged steps against time-independent weights; they equal the reference's per-step
+    Wigner transform (obs_by_adjoint = False), over two chunks and a second call."""
+    from accuratewigner_amd import WignerMoyalCUDA1D
+    kw = dict(X_gridDIM=128, X_amplitude=12., P_gridDIM=128, P_amplitude=6., dt=0.01, t=0.,
+              K="0.5 * P * P", V="-1. / sqrt(X * X + 1.37) + X * E(t)", precision=prec, batch=batch,
+              functions="__device__ double E(double t) { return 0.05 * sin(0.3 * t) * cos(0.05 * t); }",
+              diff_K="P", diff_V="X / pow(X * X + 1.37, 1.5) + E(t)",
+              abs_boundary_x="pow(fabs(sin(0.5 * M_PI * (XX + XX_max) / XX_max) * sin(0.5 * M_PI * (XX_prime + XX_prime_max) / XX_prime_max)), 0.0005)")
+    out = {}
+    for fast in (True, False):
+        g = WignerMoyalCUDA1D(**kw)
+        g.obs_by_adjoint = fast
+        W0 = np.exp(-(g.X - 1.) ** 2 - (g.P - 0.3) ** 2) / np.pi
+        W1 = np.exp(-(g.X + 0.5) ** 2 - (g.P + 0.2) ** 2) / np.pi
+        g.set_wignerfunction(np.stack([W0, W1]) if batch == 2 else W0)
+        g.propagate(300)
+        g.propagate(20)
+        assert (g._ehr["tsep"] is not None) == fast
+        out[fast] = (g.wignerfunction.get().astype(np.complex128),
+                     {n: np.array(getattr(g, n)) for n in ("X_average", "P_average", "X_average_RHS",
+                                                            "P_average_RHS", "hamiltonian_average")})
+    tol = 1e-10 if prec == "fp64" else 2e-5
+    (W1, l1), (W0_, l0) = out[True], out[False]
+    assert np.linalg.norm(W1 - W0_) / np.linalg.norm(W0_) < tol
+    for name in l0:
+        assert l1[name].shape == l0[name].shape
+        assert np.max(np.abs(l1[name] - l0[name])) < tol * max(1.0, np.max(np.abs(l0[name]))), name
+
+
+def test_wm_non_separable_time_observable_falls_back():
+    """An observable that is not a combination of a few time-independent functions (here
+    cos(X t)) keeps the per-step Wigner transform, with the same lists as the oracle."""
+    from accuratewigner_amd import WignerMoyalCUDA1D
+    g = WignerMoyalCUDA1D(X_gridDIM=64, X_amplitude=8., P_gridDIM=64, P_amplitude=8., dt=0.01, t=0.,
+                          K="0.5 * P * P", V="0.5 * X * X", diff_K="P", diff_V="X + 0.01 * cos(X * t)")
+    o = WignerMoyalOracle(X_gridDIM=64, X_amplitude=8., P_gridDIM=64, P_amplitude=8., dt=0.01,
+                          V=lambda X, t: 0.5 * X * X, K=lambda P, t: 0.5 * P * P,
+                          diff_K=lambda X, P, t: P, diff_V=lambda X, P, t: X + 0.01 * np.cos(X * t))
+    g.set_wignerfunction("exp(-pow(X - 1., 2) - pow(P, 2)) / M_PI")
+    o.set_wignerfunction(lambda X, P: np.exp(-(X - 1.) ** 2 - P ** 2) / np.pi)
+    g.propagate(40)
+    o.propagate(40)
+    assert g._ehr["tsep"] is None
+    for name in ("X_average", "P_average", "X_average_RHS", "P_average_RHS", "hamiltonian_average"):
+        a, b = np.array(getattr(g, name)), np.array(getattr(o, name))
+        assert np.max(np.abs(a - b)) < 1e-10 * max(1.0, np.max(np.abs(b))), name
