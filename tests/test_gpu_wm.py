@@ -356,3 +356,34 @@ def test_wm_non_separable_time_observable_falls_back():
     for name in ("X_average", "P_average", "X_average_RHS", "P_average_RHS", "hamiltonian_average"):
         a, b = np.array(getattr(g, name)), np.array(getattr(o, name))
         assert np.max(np.abs(a - b)) < 1e-10 * max(1.0, np.max(np.abs(b))), name
+
+
+@pytest.mark.parametrize("case", ["static_V", "tdep_mask"])
+def test_wm_time_separable_observables_other_cases(case):
+    """The time-separable path with a time-independent V (constant factor vectors, only an
+    observable depends on t) and with a time-dependent separable mask (per-step mask lines in
+    the closing vectors): equal to the per-step Wigner transform."""
+    from accuratewigner_amd import WignerMoyalCUDA1D
+    kw = dict(X_gridDIM=128, X_amplitude=10., P_gridDIM=128, P_amplitude=8., dt=0.01, t=0.,
+              K="0.5 * P * P", diff_K="P")
+    if case == "static_V":
+        kw.update(V="0.5 * X * X + 0.01 * X * X * X * X", diff_V="X + 0.04 * X * X * X + 0.001 * sin(t)",
+                  abs_boundary_x="pow(fabs(cos(0.5 * M_PI * XX / 20.)), 0.002)")
+    else:
+        kw.update(V="0.5 * X * X + 0.1 * X * sin(t)", diff_V="X + 0.1 * sin(t)",
+                  abs_boundary_x="pow(fabs(cos(0.5 * M_PI * XX / 20.) * cos(0.5 * M_PI * XX_prime / 30.)), "
+                                 "0.02 * (1. + sin(t)))")
+    out = {}
+    for fast in (True, False):
+        g = WignerMoyalCUDA1D(**kw)
+        g.obs_by_adjoint = fast
+        assert g._static == (case == "static_V")
+        g.set_wignerfunction("exp(-pow(X - 1., 2) - pow(P - 0.5, 2)) / M_PI")
+        g.propagate(270)
+        assert (g._ehr["tsep"] is not None) == fast
+        out[fast] = (g.wignerfunction.get(), {n: np.array(getattr(g, n)) for n in (
+            "X_average", "P_average", "X_average_RHS", "P_average_RHS", "hamiltonian_average")})
+    (W1, l1), (W0, l0) = out[True], out[False]
+    assert np.linalg.norm(W1 - W0) / np.linalg.norm(W0) < 1e-10
+    for name in l0:
+        assert np.max(np.abs(l1[name] - l0[name])) < 1e-10 * max(1.0, np.max(np.abs(l0[name]))), name
