@@ -149,6 +149,8 @@ static int pass_run_n(aw_plan *p, void *data, const aw_pass *ps, int nbatch, aw_
     const bool strided = ps->axis == 0;
     const int n = strided ? p->rows : p->cols;
     a.nlines = strided ? p->cols : p->rows;
+    a.nl_shift = 0;
+    while ((1 << a.nl_shift) < a.nlines) ++a.nl_shift;
     a.line_stride = strided ? 1 : p->cols;
     a.elem_stride = strided ? p->cols : 1;
     a.total_lines = (long long)a.nlines * nbatch;

@@ -19,7 +19,8 @@ struct PassArgs {
     long long line_stride;  // elements between consecutive lines
     long long elem_stride;  // elements between consecutive elements of a line
     long long total_lines;  // nlines * batch
-    int nlines;             // lines per batch item
+    int nlines;             // lines per batch item (a power of two, like every plan axis)
+    int nl_shift;           // log2(nlines)
     int dir_a, dir_b;
     int sign_in, sign_out, out_mode;
     int has_factor;

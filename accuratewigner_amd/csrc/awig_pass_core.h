@@ -174,6 +174,16 @@ template <bool STRIDED, int N, int LPB, int PADC = 4, int PADR = 5> struct LdsLa
         if constexpr (STRIDED) return (p + (p >> PADC)) * LPB + l;
         else return l * REGION + p + (p >> PADR);
     }
+    // at(p + q, l) - at(p, l) for a compile-time q, exact whenever the low PAD bits of p and
+    // q add without a carry.  The exchanges only form such sums: a thread's reads p = u,
+    // q = T k (u < T, both powers of two) and a butterfly's writes p = (b - k) RAD + k,
+    // q = r NS (k < NS, (b - k) RAD a multiple of NS RAD).  A per-thread base plus this
+    // offset lets the compiler encode the offsets in the LDS instructions instead of keeping
+    // one address register per element.
+    static constexpr int off(int q) {
+        if constexpr (STRIDED) return (q + (q >> PADC)) * LPB;
+        else return q + (q >> PADR);
+    }
 };
 
 // Stockham stage with NS = product of the radices already applied.  Thread u owns the
@@ -214,15 +224,15 @@ __device__ __forceinline__ void stockham(T2 *v, T2 *L, int l, int u, const T2 *_
 #pragma unroll
             for (int r = 0; r < RAD; ++r) v[m + r * M] = x[r];
         } else {
-            const int base = (b - k) * RAD + k;
+            const int wb = LAY::at((b - k) * RAD + k, l);
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) L[LAY::at(base + r * NS, l)] = x[r];
+            for (int r = 0; r < RAD; ++r) L[wb + LAY::off(r * NS)] = x[r];
         }
     }
     if constexpr (!LAST && !XS) {
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < E; ++kk) v[kk] = L[LAY::at(u + T * kk, l)];
+        for (int kk = 0; kk < E; ++kk) v[kk] = L[LAY::at(u, l) + LAY::off(T * kk)];
         __syncthreads();
         stockham<T2, N, E, NS * RAD, DIR, LAY, TWM, XS>(v, L, l, u, tw);
     } else if constexpr (!LAST) {
@@ -231,24 +241,24 @@ __device__ __forceinline__ void stockham(T2 *v, T2 *L, int l, int u, const T2 *_
         // real parts: outputs (b/NS)*NS*RAD + b%NS + r*NS of butterfly b = u + T*m sit in v[m + r*M]
 #pragma unroll
         for (int m = 0; m < M; ++m) {
-            const int b = u + T * m, k = b & (NS - 1), base = (b - k) * RAD + k;
+            const int b = u + T * m, k = b & (NS - 1), wb = LAY::at((b - k) * RAD + k, l);
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) Lr[LAY::at(base + r * NS, l)] = v[m + r * M].x;
+            for (int r = 0; r < RAD; ++r) Lr[wb + LAY::off(r * NS)] = v[m + r * M].x;
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < E; ++kk) v[kk].x = Lr[LAY::at(u + T * kk, l)];
+        for (int kk = 0; kk < E; ++kk) v[kk].x = Lr[LAY::at(u, l) + LAY::off(T * kk)];
         __syncthreads();
         // imaginary parts (the .y of every v[m + r*M] is still the butterfly output)
 #pragma unroll
         for (int m = 0; m < M; ++m) {
-            const int b = u + T * m, k = b & (NS - 1), base = (b - k) * RAD + k;
+            const int b = u + T * m, k = b & (NS - 1), wb = LAY::at((b - k) * RAD + k, l);
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) Lr[LAY::at(base + r * NS, l)] = v[m + r * M].y;
+            for (int r = 0; r < RAD; ++r) Lr[wb + LAY::off(r * NS)] = v[m + r * M].y;
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < E; ++kk) v[kk].y = Lr[LAY::at(u + T * kk, l)];
+        for (int kk = 0; kk < E; ++kk) v[kk].y = Lr[LAY::at(u, l) + LAY::off(T * kk)];
         __syncthreads();
         stockham<T2, N, E, NS * RAD, DIR, LAY, TWM, XS>(v, L, l, u, tw);
     }
@@ -336,8 +346,8 @@ __device__ __forceinline__ void store_block_partials(const double (&acc)[8], int
         double s = 0.0;
         for (int q = 0; q < C::THREADS / 64; ++q) s += ored[q][tid];
         const long long g0 = blk * LPB;
-        const long long bb = g0 / nl;
-        const long long bi = (g0 - bb * nl) / LPB;
+        const long long bb = g0 >> a.nl_shift;
+        const long long bi = (g0 & (nl - 1)) / LPB;
         a.obs_partials[(bb * (nl / LPB) + bi) * nc + tid] = s;
     }
 }
@@ -558,8 +568,8 @@ void k_line_pass(PassArgs a) {
     const bool active = gline < a.total_lines;
     if (!active) gline = a.total_lines - 1;   // compute on a valid line, store nothing
     const int nl = a.nlines;
-    const long long b = gline / nl;
-    const int line = int(gline - b * nl);
+    const long long b = gline >> a.nl_shift;
+    const int line = int(gline & (nl - 1));
     T2 *L = smem;
     const T2 *tw = reinterpret_cast<const T2 *>(a.tw);
     if constexpr (C::TWM != 0) {
@@ -595,7 +605,7 @@ void k_line_pass(PassArgs a) {
             }
         }
         T2 *el = smem + C::OBS3_WOFF + kVnMaxNw * N * int(sizeof(double2) / sizeof(T2));
-        const long long bi = (blk * LPB) / a.nlines;   // the block's item
+        const long long bi = (blk * LPB) >> a.nl_shift;   // the block's item
         const T2 *ef = reinterpret_cast<const T2 *>(a.obs_ef) + bi * a.ef_bstride;
 #pragma unroll
         for (int q0 = 0; q0 < N; q0 += C::THREADS)
