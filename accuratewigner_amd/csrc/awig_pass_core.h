@@ -101,63 +101,116 @@ constexpr CS root_cs(long long k, long long n) {
     return r;
 }
 
-// multiply by exp(DIR * 2 pi i K / LEN), all arguments compile-time
+template <typename R> __device__ __forceinline__ R fmar(R a, R b, R c) {
+    if constexpr (sizeof(R) == 4) return __builtin_fmaf(a, b, c);
+    else return __builtin_fma(a, b, c);
+}
+
+// In-register butterfly a <- a + w b, b <- a - w b with the compile-time root
+// w = exp(DIR 2 pi i K / LEN).  Products by 1, -1 and +-i are exact; otherwise
+// w b = c (b.x - t b.y, b.y + t b.x) with c = Re w and t = Im w / Re w, which makes the
+// butterfly six fused multiply-adds instead of a complex product and four additions
+// (Goedecker's rotation form; |t| <= tan(3 pi / 8) on the roots used here).
 template <int DIR, int K, int LEN, typename T2>
-__device__ __forceinline__ T2 rot(T2 d) {
-    if constexpr (K == 0) {
-        return d;
-    } else if constexpr (4 * K == LEN) {
-        T2 r;
-        if constexpr (DIR < 0) { r.x = d.y; r.y = -d.x; }
-        else { r.x = -d.y; r.y = d.x; }
-        return r;
+__device__ __forceinline__ void bfly(T2 &a, T2 &b) {
+    constexpr int Kn = ((K % LEN) + LEN) % LEN;
+    using R = decltype(a.x);
+    T2 p, m;
+    if constexpr (Kn == 0) {
+        p = cadd(a, b);
+        m = csub(a, b);
+    } else if constexpr (2 * Kn == LEN) {
+        p = csub(a, b);
+        m = cadd(a, b);
+    } else if constexpr (4 * Kn == LEN || 4 * Kn == 3 * LEN) {
+        // w = sg i, w b = sg (-b.y, b.x)
+        constexpr int sg = (4 * Kn == LEN) ? DIR : -DIR;
+        if constexpr (sg > 0) {
+            p.x = a.x - b.y; p.y = a.y + b.x;
+            m.x = a.x + b.y; m.y = a.y - b.x;
+        } else {
+            p.x = a.x + b.y; p.y = a.y - b.x;
+            m.x = a.x - b.y; m.y = a.y + b.x;
+        }
     } else {
-        constexpr CS w = root_cs(K, LEN);
-        using R = decltype(d.x);
+        constexpr CS w = root_cs(Kn, LEN);
         constexpr R c = static_cast<R>(w.c);
-        constexpr R s = static_cast<R>(DIR * w.s);
-        T2 r;
-        r.x = d.x * c - d.y * s;
-        r.y = d.x * s + d.y * c;
-        return r;
+        constexpr R t = static_cast<R>(DIR * w.s / w.c);
+        const R u = fmar(-t, b.y, b.x);
+        const R v = fmar(t, b.x, b.y);
+        p.x = fmar(c, u, a.x); p.y = fmar(c, v, a.y);
+        m.x = fmar(-c, u, a.x); m.y = fmar(-c, v, a.y);
+    }
+    a = p;
+    b = m;
+}
+
+// X[k], X[k + N/2] = E[k] +- w^k O[k] for k = K .. N/2 - 1 (w = exp(DIR 2 pi i / N)), in place
+template <int DIR, int N, int K, typename T2>
+__device__ __forceinline__ void dit_combine(T2 *e, T2 *o) {
+    if constexpr (K < N / 2) {
+        bfly<DIR, K, N>(e[K], o[K]);
+        dit_combine<DIR, N, K + 1>(e, o);
     }
 }
 
-// in-register radix-2 decimation-in-frequency DFT of size N (natural order in and out)
-template <int DIR, int N, int LEN, int K, typename T2>
-__device__ __forceinline__ void dif_k(T2 *x) {
-    if constexpr (K < LEN / 2) {
-#pragma unroll
-        for (int s = 0; s < N; s += LEN) {
-            T2 a = x[s + K], b = x[s + K + LEN / 2];
-            x[s + K] = cadd(a, b);
-            x[s + K + LEN / 2] = rot<DIR, K, LEN>(csub(a, b));
-        }
-        dif_k<DIR, N, LEN, K + 1>(x);
-    }
-}
-template <int DIR, int N, int LEN, typename T2>
-__device__ __forceinline__ void dif_levels(T2 *x) {
-    if constexpr (LEN >= 2) {
-        dif_k<DIR, N, LEN, 0>(x);
-        dif_levels<DIR, N, LEN / 2>(x);
-    }
-}
-constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
-constexpr int bitrev(int i, int bits) {
-    int r = 0;
-    for (int b = 0; b < bits; ++b) r |= ((i >> b) & 1) << (bits - 1 - b);
-    return r;
-}
+// In-register DFT of size N (a power of two, natural order in and out), X[k] = sum_n x[n]
+// exp(DIR 2 pi i n k / N), decimated in time so that every product by a root of unity is a
+// butterfly above: N = 16 as 4 x 4 (n = 4 n1 + n2, k = k1 + 4 k2; the twiddles w^(n2 k1)
+// are pushed into the second-level butterflies), other sizes by even / odd halves.
 template <int DIR, int N, typename T2>
 __device__ __forceinline__ void dft_reg(T2 *x) {
-    if constexpr (N > 1) {
-        dif_levels<DIR, N, N>(x);
-        T2 y[N];
+    if constexpr (N == 2) {
+        bfly<DIR, 0, 2>(x[0], x[1]);
+    } else if constexpr (N == 4) {
+        bfly<DIR, 0, 4>(x[0], x[2]);      // E = (x0 + x2, x0 - x2)
+        bfly<DIR, 0, 4>(x[1], x[3]);      // O = (x1 + x3, x1 - x3)
+        bfly<DIR, 0, 4>(x[0], x[1]);      // X0, X2
+        bfly<DIR, 1, 4>(x[2], x[3]);      // X1, X3
+        const T2 t = x[1];
+        x[1] = x[2];
+        x[2] = t;
+    } else if constexpr (N == 16) {
+        T2 a[4][4];
 #pragma unroll
-        for (int i = 0; i < N; ++i) y[i] = x[bitrev(i, ilog2(N))];
+        for (int n2 = 0; n2 < 4; ++n2) {
 #pragma unroll
-        for (int i = 0; i < N; ++i) x[i] = y[i];
+            for (int n1 = 0; n1 < 4; ++n1) a[n2][n1] = x[4 * n1 + n2];
+            dft_reg<DIR, 4>(a[n2]);
+        }
+        // for each k1: X[k1 + 4 k2] = DFT4 over n2 of w^(n2 k1) a[n2][k1], w = exp(DIR 2 pi i / 16):
+        //   c0, c1 = a0 +- w^2k1 a2;  d0, d1 = a1 +- w^2k1 a3;
+        //   X[k1], X[k1 + 8] = c0 +- w^k1 d0;  X[k1 + 4], X[k1 + 12] = c1 +- w^(k1 + 4) d1
+        bfly<DIR, 0, 16>(a[0][0], a[2][0]); bfly<DIR, 0, 16>(a[1][0], a[3][0]);
+        bfly<DIR, 0, 16>(a[0][0], a[1][0]); bfly<DIR, 4, 16>(a[2][0], a[3][0]);
+        bfly<DIR, 2, 16>(a[0][1], a[2][1]); bfly<DIR, 2, 16>(a[1][1], a[3][1]);
+        bfly<DIR, 1, 16>(a[0][1], a[1][1]); bfly<DIR, 5, 16>(a[2][1], a[3][1]);
+        bfly<DIR, 4, 16>(a[0][2], a[2][2]); bfly<DIR, 4, 16>(a[1][2], a[3][2]);
+        bfly<DIR, 2, 16>(a[0][2], a[1][2]); bfly<DIR, 6, 16>(a[2][2], a[3][2]);
+        bfly<DIR, 6, 16>(a[0][3], a[2][3]); bfly<DIR, 6, 16>(a[1][3], a[3][3]);
+        bfly<DIR, 3, 16>(a[0][3], a[1][3]); bfly<DIR, 7, 16>(a[2][3], a[3][3]);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            x[k1] = a[0][k1];
+            x[k1 + 8] = a[1][k1];
+            x[k1 + 4] = a[2][k1];
+            x[k1 + 12] = a[3][k1];
+        }
+    } else if constexpr (N > 1) {
+        T2 e[N / 2], o[N / 2];
+#pragma unroll
+        for (int n = 0; n < N / 2; ++n) {
+            e[n] = x[2 * n];
+            o[n] = x[2 * n + 1];
+        }
+        dft_reg<DIR, N / 2>(e);
+        dft_reg<DIR, N / 2>(o);
+        dit_combine<DIR, N, 0>(e, o);
+#pragma unroll
+        for (int k = 0; k < N / 2; ++k) {
+            x[k] = e[k];
+            x[k + N / 2] = o[k];
+        }
     }
 }
 
