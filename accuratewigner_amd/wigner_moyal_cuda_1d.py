@@ -197,6 +197,21 @@ extern "C" __global__ void aw_eval_mask_lines(int which, const double *xj, int n
         }
     }
 }
+// the mask table at the time *tp (read on the device, so that captured step loops can run it),
+// in FP64 (out64) or FP32 (out32)
+extern "C" __global__ void aw_eval_mask_at(int which, const double *xj, int nj, const double *xi, int ni,
+                                           const double *tp, double *out64, float *out32)
+{
+    const long long total = (long long)ni * nj;
+    const double t = *tp;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const size_t i = idx / nj, j = idx %% nj;
+        const double m = which == 0 ? aw_absx(xj[j], xi[i], t, i, j) : aw_absp(xj[j], xi[i], t, i, j);
+        if (out64) out64[idx] = m;
+        else out32[idx] = (float)m;
+    }
+}
 extern "C" __global__ void aw_eval_mask(int which, const double *xj, int nj, const double *xi,
                                         int ni, double t, double *out)
 {
@@ -235,10 +250,11 @@ extern "C" __global__ void aw_eval_mask(int which, const double *xj, int nj, con
     def _prepare_masks(self):
         """Evaluate abs_boundary_x(XX, XX', t) and abs_boundary_p(PP, PP', t), which expV /
         expK apply at the step time t (:753, :777).  A time-independent mask is evaluated once:
-        two vectors when it is separable, else a 2D table.  A time-dependent mask must stay
-        separable (rank 1 at every sampled time): its line and element vectors are evaluated
-        on the device for every step of a chunk (``_mask_tables``) and multiply the factor
-        vectors; a time-dependent mask that is not separable raises NotImplementedError."""
+        two vectors when it is separable, else a 2D table.  A time-dependent mask that stays
+        separable (rank 1 at every sampled time) has its line and element vectors evaluated on
+        the device for every step of a chunk (``_mask_tables``), which multiply the factor
+        vectors.  Any other time-dependent mask is evaluated as a 2D table at each step's time
+        (``_ttable``) and its steps run unmerged, one step of three passes at a time."""
         self._masks = {}
         for which, expr in ((0, self.abs_boundary_x), (1, self.abs_boundary_p)):
             if expr.strip() in ("1.", "1", "1.0"):
@@ -248,17 +264,12 @@ extern "C" __global__ void aw_eval_mask(int which, const double *xj, int nj, con
             M1 = self._eval_mask(which, float(self.t) + 1.2345)
             if not np.array_equal(M0, M1):
                 M2 = self._eval_mask(which, float(self.t) + 2.7183)
-                if any(rank1_split(M) is None for M in (M0, M1, M2)):
-                    raise NotImplementedError(
-                        "time-dependent absorbing boundaries must be separable, f(XX, t) g(XX', t) "
-                        "(%r)" % expr)
                 # pivot of the split: the largest entry, where the mask is nearest 1
                 i0, j0 = np.unravel_index(np.argmax(np.abs(M0)), M0.shape)
-                if any(M[i0, j0] == 0.0 for M in (M1, M2)):
-                    raise NotImplementedError(
-                        "time-dependent absorbing boundary %r vanishes at its largest entry (%d, %d) "
-                        "at a later time" % (expr, i0, j0))
-                self._masks[which] = ("time", int(i0), int(j0))
+                if any(rank1_split(M) is None for M in (M0, M1, M2)) or any(M[i0, j0] == 0.0 for M in (M1, M2)):
+                    self._masks[which] = ("ttable",)
+                else:
+                    self._masks[which] = ("time", int(i0), int(j0))
                 continue
             split = rank1_split(M0)
             if split is not None:
@@ -640,7 +651,60 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         return ts
 
     def _tdep_masks(self):
-        return any(m is not None and m[0] == "time" for m in self._masks.values())
+        return any(m is not None and m[0] in ("time", "ttable") for m in self._masks.values())
+
+    def _ttables(self):
+        """Masks evaluated as 2D tables at every step's time (time-dependent, not separable)."""
+        return any(m is not None and m[0] == "ttable" for m in self._masks.values())
+
+    def _ttable(self, which, t_ptr):
+        """The table of mask ``which`` at the time stored at device address t_ptr, in the plan's
+        real type, in a persistent buffer (one launch, capturable), or None for other masks."""
+        m = self._masks.get(which)
+        if m is None or m[0] != "ttable":
+            return None
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        out = self._ttable_buf(which)
+        xj, xi = (self._dev_kXX, self._dev_kXXp) if which == 0 else (self._dev_kPP, self._dev_kPPp)
+        f64 = np.dtype(self.plan.real_dtype) == np.float64
+        blocks, threads = rtc.launch_cfg(Nx * Np)
+        self._mod.launch("aw_eval_mask_at", (blocks,), (threads,),
+                         [ctypes.c_int(which), xj.ptr, ctypes.c_int(Nx), xi.ptr, ctypes.c_int(Np),
+                          ctypes.c_void_p(t_ptr), ctypes.c_void_p(out.ptr if f64 else None),
+                          ctypes.c_void_p(None if f64 else out.ptr)], current_stream())
+        return out
+
+    def _ttable_buf(self, which):
+        """Persistent buffer of a per-step mask table (allocated before any capture uses it)."""
+        tabs = self.__dict__.setdefault("_tt_bufs", {})
+        if which not in tabs:
+            tabs[which] = empty((self.P_gridDIM, self.X_gridDIM), self.plan.real_dtype)
+        return tabs[which]
+
+    def _alloc_ttables(self):
+        for which, m in self._masks.items():
+            if m is not None and m[0] == "ttable":
+                self._ttable_buf(which)
+
+    def _ttable_chunk(self, n):
+        """n unmerged steps [EV, I0] [F1, EK, I1] [F0, EV] from the chunk buffers, each with its
+        mask tables evaluated at the step's time (the time expV / expK see, :753, :777)."""
+        colL, colE, rowL, rowE = self._prop["fac"]
+        ts = self._prop["ts"]
+        csz = self.plan.fsize
+        Nx, Np = self.X_gridDIM, self.P_gridDIM
+        for k in range(n):
+            tx, tp = self._step_tables(ts.ptr + 8 * k)
+            self.plan.steps(self.rho, 1, colL.ptr + 2 * k * Nx * csz, colE.ptr + 2 * k * Np * csz,
+                            rowL.ptr + k * Np * csz, rowE.ptr + k * Nx * csz, mask_open_close=tx, mask_row=tp)
+
+    def _step_tables(self, t_ptr):
+        """(coordinate mask, momentum mask) tables of one unmerged step at the time at t_ptr:
+        the static table of a mask, its table at that time, or None."""
+        tx = self._ttable(0, t_ptr)
+        tp = self._ttable(1, t_ptr)
+        return (tx if tx is not None else self._mask_args(0, 1)[2],
+                tp if tp is not None else self._mask_args(1, 1)[2])
 
     def _mask_tables(self, ts_dev, n, out=None):
         """Per-step vectors of the time-dependent masks at the step times ts_dev[:n] (t after
@@ -663,7 +727,7 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
 
     def _mask_args(self, which, power):
         m = self._masks.get(which)
-        if m is None or m[0] == "time":
+        if m is None or m[0] in ("time", "ttable"):
             return None, None, None
         if m[0] == "vec":
             return m[1], m[2], None          # (v over lines-i, u over cols-j)
@@ -858,6 +922,13 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         _, _, tab1 = self._mask_args(0, 1)
         _, _, tab2 = self._mask_args(0, 2)
         _, _, tabp = self._mask_args(1, 1)
+        if self._ttables():
+            # one step (single_step_propagation): the mask tables at its time
+            if n != 1:
+                raise ValueError("steps with tabulated time-dependent masks run one at a time")
+            self._alloc_ttables()
+            tdev = to_device(np.asarray(ts, np.float64))
+            tab1, tabp = self._step_tables(tdev.ptr)
         p, rho = self.plan, self.rho
         ev = self._pass_events
         if ev is not None:
@@ -893,6 +964,16 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
             CH = CHUNK_STEPS
             if self._prop is None:
                 self._prop = dict(self._chunk_bufs(CH), graphs={})
+            if self._ttables():
+                # masks tabulated at every step's time: unmerged steps, three passes each
+                self._alloc_ttables()
+                for c0 in range(0, steps, CH):
+                    n = min(CH, steps - c0)
+                    self._build_chunk_factors(ts_all.tensor[c0:c0 + n], merged=False, bufs=self._prop)
+                    self._timed_passes(lambda n=n: self._ttable_chunk(n), 3 * n)
+                self.t = ts[-1]
+                self.normalize_rho_wigner()
+                return self.wignerfunction
             # latency-bound sizes (the small-pass kernels) replay a captured chunk
             small = self.batch * max(self.X_gridDIM, self.P_gridDIM) <= 2048
             normalized = False
@@ -934,6 +1015,7 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
                              graphs={})
             self._observable_kernel(exprs)
             self._obs_partials_for(nobs)             # allocated here, never inside a capture
+            self._alloc_ttables()
             # (time-dependent masks change the closing half-kick every step: no merged steps)
             g = self._obs_weights(exprs) if (self.obs_by_adjoint and not self._tdep_masks()) else None
             # time-independent observables imply a time-independent V (the Hamiltonian is one of
@@ -945,7 +1027,8 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
                 nblk = self.plan.row_obs_blocks()
                 self._ehr["wring"] = empty((CH, self.batch, nblk, nobs), np.float64)
                 self._ehr["wnblk"] = nblk
-            elif self.obs_by_adjoint and self._mask_args(0, 1)[2] is None and self.plan.obs_blocks(0) > 0:
+            elif (self.obs_by_adjoint and self._mask_args(0, 1)[2] is None and not self._ttables()
+                  and self.plan.obs_blocks(0) > 0):
                 # observables that depend on t through a few functions of t: merged steps whose
                 # column pass records the closed state's sums against time-independent weights
                 tsep = self._tsep_setup(exprs)
@@ -1109,10 +1192,14 @@ extern "C" __global__ void aw_init(%(st)s *W, long long n_item, long long n_tota
         nobs = len(exprs)
         row_bytes = self.batch * nobs * 8
         stage, ts = bufs["stage"], bufs["ts"]
+        ttab = self._ttables()
         for k in range(n):
             # one step = three passes; its opening / closing vectors sit at 2 k and 2 k + 1
             # (at 0 and 1 for every step with a time-independent H)
             kc = 0 if self._static else k
+            if ttab:
+                # masks tabulated at the step's time
+                tab1, tabp = self._step_tables(ts.ptr + 8 * k)
             p.steps(rho, 1, colL.ptr + 2 * kc * Nx * csz, colE.ptr + 2 * kc * Np * csz,
                     rowL.ptr + kc * Np * csz, rowE.ptr + kc * Nx * csz, mask_open_close=tab1,
                     mask_row=tabp, dscale=self._dscale)

@@ -151,11 +151,59 @@ def test_wm_time_dependent_separable_masks(ehrenfest):
             assert np.max(np.abs(a - b)) < 1e-10 * max(1.0, np.max(np.abs(b))), name
 
 
-def test_wm_time_dependent_nonseparable_mask_is_refused():
-    from accuratewigner_amd import WignerMoyalCUDA1D
-    with pytest.raises(NotImplementedError):
-        WignerMoyalCUDA1D(X_gridDIM=64, X_amplitude=5., P_gridDIM=64, P_amplitude=5., dt=0.01, t=0.,
-                          K="0.5 * P * P", V="0.5 * X * X", abs_boundary_x="1. - 0.001 * t * XX * XX_prime")
+def _ntmask_pair(ehrenfest, both, precision="fp64", batch=1):
+    """A coordinate mask that depends on t and on XX - XX' (not separable) and a momentum
+    mask that depends on t, separable (``both`` False) or not."""
+    mx = "1. - 0.002 * (1. + sin(t)) * pow(XX - XX_prime, 2) / (1. + pow(XX - XX_prime, 2))"
+    fx = lambda XX, XXp, t: 1. - 0.002 * (1. + np.sin(t)) * (XX - XXp) ** 2 / (1. + (XX - XXp) ** 2)  # noqa: E731
+    if both:
+        mp = "1. - 0.001 * t * pow(PP + PP_prime, 2) / (10. + pow(PP + PP_prime, 2))"
+        fp = lambda PP, PPp, t: 1. - 0.001 * t * (PP + PPp) ** 2 / (10. + (PP + PPp) ** 2)  # noqa: E731
+    else:
+        mp = "exp(-0.001 * (1. + t) * (PP * PP + PP_prime * PP_prime))"
+        fp = lambda PP, PPp, t: np.exp(-0.001 * (1. + t) * (PP * PP + PPp * PPp))  # noqa: E731
+    V = "0.5 * X * X + 0.1 * X * sin(t)"
+    Vf = lambda X, t: 0.5 * X * X + 0.1 * X * np.sin(t)  # noqa: E731
+    kw = dict(batch=batch) if batch > 1 else {}
+    g, o = _pair(Nx=128, Np=64, Xa=8., Pa=6., V=V, Vf=Vf, ehrenfest=ehrenfest, abs_boundary_x=mx,
+                 abs_boundary_p=mp, precision=precision, **kw)
+    o.abs_boundary_x, o.abs_boundary_p = fx, fp
+    o.set_wignerfunction(lambda X, P: np.exp(-(X - 1.) ** 2 - (P - 0.5) ** 2) / np.pi)
+    g.set_wignerfunction("exp(-pow(X - 1., 2) - pow(P - 0.5, 2)) / M_PI")
+    return g, o
+
+
+@pytest.mark.parametrize("ehrenfest,both", [(False, False), (False, True), (True, False), (True, True)])
+def test_wm_time_dependent_nonseparable_masks(ehrenfest, both):
+    """Masks that depend on t and are not separable are tabulated at every step's time, and
+    the steps run unmerged: W and the Ehrenfest lists vs the oracle (FP64, 1e-10)."""
+    g, o = _ntmask_pair(ehrenfest, both)
+    assert g._masks[0][0] == "ttable" and g._masks[1][0] == ("ttable" if both else "time")
+    steps = 300 if not ehrenfest else 260   # the Ehrenfest run crosses a full 256-step chunk
+    W = g.propagate(steps).get()
+    Wo = o.propagate(steps)
+    assert rel(W, Wo) < 1e-10
+    if ehrenfest:
+        for name in ("X_average", "P_average", "X_average_RHS", "P_average_RHS", "hamiltonian_average"):
+            a, b = np.array(getattr(g, name)), np.array(getattr(o, name))
+            assert np.max(np.abs(a - b)) < 1e-10 * max(1.0, np.max(np.abs(b))), name
+    # a later call continues from the same time; a single step evaluates the tables at t
+    assert rel(g.propagate(7).get(), o.propagate(7)) < 1e-10
+    g.single_step_propagation()
+    o.single_step_propagation()
+    assert rel(g.rho.get(), o.rho) < 1e-10
+
+
+def test_wm_time_dependent_nonseparable_masks_mixed_and_batch():
+    """The tabulated masks in mixed precision (FP32 tables) against FP64, and a batch of two
+    identical states equal to one state, item by item."""
+    g, o = _ntmask_pair(False, True, precision="mixed")
+    assert rel(g.propagate(200).get(), o.propagate(200)) < 2e-5
+    g2, o2 = _ntmask_pair(False, True, batch=2)
+    W2 = g2.propagate(120).get()
+    W1 = o2.propagate(120)
+    for b in range(2):
+        assert rel(W2[b], W1) < 1e-10
 
 
 def test_wm_nonseparable_mask_table():
