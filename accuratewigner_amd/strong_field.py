@@ -14,6 +14,8 @@ Here the same flow runs without plotting.  h5py is not available, so results go 
 memory together.  The ground-state cache is ``ground_state_rho.npy`` (no pickle).
 """
 import os
+import queue
+import threading
 import zipfile
 
 import numpy as np
@@ -94,23 +96,61 @@ def rv_ground_state_rho(params, nsteps=5000):
 
 
 class NpzWriter:
-    """Incremental ``.npz`` writer: every ``add`` appends one ``<key>.npy`` member."""
+    """Incremental ``.npz`` writer: every ``add`` appends one ``<key>.npy`` member.
 
-    def __init__(self, path):
+    With ``background=True`` the members are written by one writer thread, in the order they
+    were added, so that writing a frame overlaps the next frame's propagation (the array
+    handed to ``add`` must not be modified afterwards); ``close`` waits for the queue and
+    raises any error the thread met."""
+
+    def __init__(self, path, background=False):
         self.path = path
         self._zf = zipfile.ZipFile(path, "w", compression=zipfile.ZIP_STORED, allowZip64=True)
+        self._q = self._thread = self._err = None
+        if background:
+            self._q = queue.Queue(maxsize=8)
+            self._thread = threading.Thread(target=self._loop, name="npz-writer", daemon=True)
+            self._thread.start()
 
-    def add(self, key, value):
-        arr = np.asanyarray(value)
-        if arr.dtype == object:
-            raise TypeError("%s: not an array value" % key)
+    def _write(self, key, arr):
         with self._zf.open(key + ".npy", "w", force_zip64=True) as f:
             # (np.ascontiguousarray would turn a 0-d scalar into shape (1,))
             np.lib.format.write_array(f, arr if arr.flags.c_contiguous else np.array(arr, order="C"),
                                       allow_pickle=False)
 
+    def _loop(self):
+        while True:
+            item = self._q.get()
+            if item is None:
+                return
+            if self._err is None:
+                try:
+                    self._write(*item)
+                except BaseException as e:    # reported by the next add / close
+                    self._err = e
+
+    def _check(self):
+        if self._err is not None:
+            err, self._err = self._err, None
+            raise err
+
+    def add(self, key, value):
+        arr = np.asanyarray(value)
+        if arr.dtype == object:
+            raise TypeError("%s: not an array value" % key)
+        if self._q is None:
+            self._write(key, arr)
+        else:
+            self._check()
+            self._q.put((key, arr))
+
     def close(self):
+        if self._thread is not None:
+            self._q.put(None)
+            self._thread.join()
+            self._thread = None
         self._zf.close()
+        self._check()
 
     def __enter__(self):
         return self
@@ -168,10 +208,17 @@ class StrongFieldRun:
             writer.add("settings/P_wigner", np.broadcast_to(qs.P, qs.P.shape))
 
     def frame(self):
-        """One frame (:236-269): propagate num_iteration steps, store W, t and diag(rho)."""
+        """One frame (:236-269): propagate num_iteration steps, store W, t and diag(rho).
+        Re W is taken on the device and copied into page-locked memory (half the bytes of
+        the complex array, one DMA); a background writer then stores it while the next frame
+        runs."""
+        import torch
         qs = self.quant_sys
         qs.propagate(self.num_iteration)
-        wigner = qs.wignerfunction.get().real
+        w_dev = qs.wignerfunction.tensor.real
+        host = torch.empty(w_dev.shape, dtype=w_dev.dtype, pin_memory=True)
+        host.copy_(w_dev)
+        wigner = host.numpy()
         prob = qs.rho.tensor.diagonal().cpu().numpy()
         if self.writer is not None:
             key = "frames/%d/" % self.current_frame_num
@@ -200,7 +247,7 @@ class StrongFieldRun:
 
 
 def main(path="strong_field_physics.npz", max_frames=881):
-    with NpzWriter(path) as w:
+    with NpzWriter(path, background=True) as w:
         run = StrongFieldRun(writer=w)
         run.run(max_frames)
         run.save_ehrenfest()

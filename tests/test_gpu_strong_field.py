@@ -19,7 +19,7 @@ def test_strong_field_frames_match_oracle(tmp_path):
     p = _params()
     gs = str(tmp_path / "gs.npy")
     out = str(tmp_path / "sf.npz")
-    with NpzWriter(out) as w:
+    with NpzWriter(out, background=True) as w:
         run = StrongFieldRun(p, writer=w, ground_state_file=gs, num_iteration=20, ground_state_max_steps=200)
         assert not run.ground_state_loaded
         run.run(max_frames=3)

@@ -32,3 +32,28 @@ def test_npz_writer_layout(tmp_path):
     assert sorted(z.files) == ["frames/0/wigner", "settings/V", "settings/dt"]
     assert np.array_equal(z["frames/0/wigner"], np.arange(6.).reshape(2, 3))
     assert float(z["settings/dt"]) == 0.01 and str(z["settings/V"]) == "X * X"
+
+
+def test_npz_writer_background_thread(tmp_path):
+    """The background writer stores the members in the order they were added, and reports
+    an error of its thread at the next call."""
+    from accuratewigner_amd.strong_field import NpzWriter
+    path = str(tmp_path / "b.npz")
+    frames = [np.random.default_rng(k).normal(size=(64, 128)) for k in range(12)]
+    with NpzWriter(path, background=True) as w:
+        for k, a in enumerate(frames):
+            w.add("frames/%d/wigner" % k, a)
+            w.add("frames/%d/t" % k, np.float64(0.5 * k))
+    z = np.load(path, allow_pickle=False)
+    assert len(z.files) == 24
+    for k, a in enumerate(frames):
+        assert np.array_equal(z["frames/%d/wigner" % k], a)
+        assert float(z["frames/%d/t" % k]) == 0.5 * k
+    w = NpzWriter(str(tmp_path / "c.npz"), background=True)
+    w._zf.close()                  # a closed archive: the thread's write fails
+    w.add("x", np.zeros(3))
+    try:
+        w.close()
+        raise AssertionError("the thread's error must be raised")
+    except ValueError:
+        pass
