@@ -23,7 +23,7 @@ from types import FunctionType
 
 import numpy as np
 
-from . import _lib, rtc
+from . import _lib, graphs, rtc
 from ._params import KeywordParams
 from .device import DeviceArray, current_stream, empty, require_gpu, to_device
 from .engine import FWD, INV, Plan, mem_info
@@ -241,6 +241,41 @@ extern "C" __global__ void aw_weighted(const %(st)s *psi, int n, double t, int p
         out[threadIdx.x] = s;
     }
 }
+// the same sums as block partials of a grid-stride loop, the time read from device memory
+// (ts): partials[block][q]; aw_weighted_fin adds the blocks in order into out[q]
+extern "C" __global__ void aw_weighted_mb(const %(st)s *psi, int n, const double *ts, int perm_n2,
+                                          double *partials)
+{
+    const double t = *ts;
+    double acc[%(nobs)d];
+    for (int q = 0; q < %(nobs)d; ++q) acc[q] = 0.;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const int kp = perm_n2 > 0 ? (j / perm_n2) + (n / perm_n2) * (j %% perm_n2) : j;
+        const double X = dX * (j - 0.5 * X_gridDIM);
+        const double P = dP * ((kp + X_gridDIM / 2) %% X_gridDIM - 0.5 * X_gridDIM);
+        const double w = (double)psi[j].x * psi[j].x + (double)psi[j].y * psi[j].y;
+%(body)s    }
+    __shared__ double red[4][%(nobs)d];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int c = 0; c < %(nobs)d; ++c) {
+        const double s = aw_wave_sum(acc[c]);
+        if (lane == 0) red[wid][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < %(nobs)d) {
+        double s = 0.;
+        for (int w = 0; w < 4; ++w) s += red[w][threadIdx.x];
+        partials[blockIdx.x * %(nobs)d + threadIdx.x] = s;
+    }
+}
+extern "C" __global__ void aw_weighted_fin(const double *partials, int nblk, double *out)
+{
+    if (threadIdx.x < %(nobs)d) {
+        double s = 0.;
+        for (int b = 0; b < nblk; ++b) s += partials[b * %(nobs)d + threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
 extern "C" __global__ void aw_inv_norm(const double *stats, double factor, double *dscale)
 {
     dscale[0] = 1.0 / (sqrt(stats[2]) * factor);
@@ -259,6 +294,21 @@ extern "C" __global__ void aw_inv_norm(const double *stats, double factor, doubl
         mod.launch("aw_weighted", (1,), (256,), [psi.ptr, ctypes.c_int(self.X_gridDIM),
                                                 ctypes.c_double(float(self.t if t is None else t)),
                                                 ctypes.c_int(perm), out_ptr],
+                   current_stream())
+
+    def _wblocks(self):
+        return max(1, min(64, self.X_gridDIM // 1024))
+
+    def _weighted_dev(self, exprs, out_ptr, t_ptr, partials, psi=None, p_rep=False):
+        """``_weighted`` over several blocks (then one that adds them in order), with the time
+        read from device memory: a fixed launch pair that a captured loop can replay."""
+        psi = self.wavefunction if psi is None else psi
+        mod = self._obs_module(exprs)
+        perm = self._two[1] if (p_rep and self._two is not None) else 0
+        nblk = self._wblocks()
+        mod.launch("aw_weighted_mb", (nblk,), (256,), [psi.ptr, ctypes.c_int(self.X_gridDIM), ctypes.c_void_p(t_ptr),
+                                                      ctypes.c_int(perm), partials.ptr], current_stream())
+        mod.launch("aw_weighted_fin", (1,), (64,), [partials.ptr, ctypes.c_int(nblk), ctypes.c_void_p(out_ptr)],
                    current_stream())
 
     def _fft(self, data, direction, out=None):
@@ -372,41 +422,15 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
         p.factors(ek, N, n, A=Kt.ptr, ca=-dt, mag=bp, mode=mode, scale=1.0 if raw else 1.0 / N)
         return ev, ek
 
-    def _run(self, ts, bloch=False, per_step=None, raw=False):
-        """Steps at times ts; merged half-kicks unless ``per_step`` needs psi after each step."""
+    def _run(self, ts, bloch=False, raw=False):
+        """Steps at times ts with merged half-kicks (2 n + 1 passes)."""
         n = len(ts)
         N = self.X_gridDIM
         csz = self.plan.fsize
         p, psi = self.plan, self.wavefunction
+        ev, ek = self._factors(ts, bloch, raw)
         if self._two is not None:
-            self._run_two(ts, bloch, per_step, raw)
-            return
-        if per_step is None:
-            ev, ek = self._factors(ts, bloch, raw)
-            p.run(psi, 1, 0, FWD, elem_fac=ev.ptr, dscale=self._dscale if self._scale_pending else None)
-            self._scale_pending = False
-            for s in range(n):
-                p.run(psi, 1, 0, INV, elem_fac=ek.ptr + s * N * csz)
-                if s < n - 1:
-                    p.run(psi, 1, 0, FWD, elem_fac=ev.ptr + (s + 1) * N * csz)
-            p.run(psi, 1, 0, 0, elem_fac=ev.ptr + n * N * csz)
-            return
-        for s in range(n):
-            ev, ek = self._factors(ts[s:s + 1], bloch)
-            p.run(psi, 1, 0, FWD, elem_fac=ev.ptr, dscale=self._dscale if self._scale_pending else None)
-            self._scale_pending = False
-            p.run(psi, 1, 0, INV, elem_fac=ek.ptr)
-            p.run(psi, 1, 0, 0, elem_fac=ev.ptr + N * csz)
-            per_step(s)
-
-    def _run_two(self, ts, bloch, per_step, raw):
-        """_run for two-level lines: EV and EK are full-line tables applied inside the passes."""
-        n = len(ts)
-        N = self.X_gridDIM
-        csz = self.plan.fsize
-        psi = self.wavefunction
-        if per_step is None:
-            ev, ek = self._factors(ts, bloch, raw)
+            # two-level lines: EV and EK are full-line tables applied inside the passes
             self._col(psi, 0, FWD, ctab=ev.ptr, twiddle=True,
                       dscale=self._dscale if self._scale_pending else None)
             self._scale_pending = False
@@ -415,10 +439,13 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
                 last = s == n - 1
                 self._col(psi, INV, 0 if last else FWD, ctab=ev.ptr + (s + 1) * N * csz, twiddle=not last)
             return
+        p.run(psi, 1, 0, FWD, elem_fac=ev.ptr, dscale=self._dscale if self._scale_pending else None)
+        self._scale_pending = False
         for s in range(n):
-            ev, ek = self._factors(ts[s:s + 1], bloch)
-            self._two_step(ev, ek)
-            per_step(s)
+            p.run(psi, 1, 0, INV, elem_fac=ek.ptr + s * N * csz)
+            if s < n - 1:
+                p.run(psi, 1, 0, FWD, elem_fac=ev.ptr + (s + 1) * N * csz)
+        p.run(psi, 1, 0, 0, elem_fac=ev.ptr + n * N * csz)
 
     def _two_step(self, ev, ek):
         """One unmerged two-level step (three passes) from factor tables ev[0..1], ek[0]."""
@@ -448,22 +475,20 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
                 self._normalize(np.sqrt(self.dX))
             self.t = ts[-1]
             return self
-        N = self.X_gridDIM
-        xs = ["1.", "X", self.diff_V, self.V]
-        ps = ["1.", self.diff_K, "P", self.K]
+        N, CH = self.X_gridDIM, CHUNK_STEPS
         ring = empty((steps, 8), np.float64)
-
-        def observe(s, c0):
-            k = c0 + s
-            t = float(ts[k])
-            self._weighted(xs, ring.ptr + k * 64, t=t)
-            self._fft(self.wavefunction, FWD, out=self._tmp)
-            self._weighted(ps, ring.ptr + k * 64 + 32, psi=self._tmp, t=t, p_rep=True)
-            self._norm_scale(np.sqrt(self.dX))
+        b = self._ehr_bufs()
+        ts_all = to_device(ts)          # one upload; each chunk copies its slice on the device
+        if not self._scale_pending:
+            # every step applies the pending scale: start from 1 so that all chunks are alike
+            self._dscale.tensor[:, 0] = 1.0
+            self._dscale.tensor[:, 1] = 0.0
             self._scale_pending = True
-
-        for c0 in range(0, steps, CHUNK_STEPS):
-            self._run(ts[c0:c0 + CHUNK_STEPS], per_step=lambda s, c0=c0: observe(s, c0))
+        for c0 in range(0, steps, CH):
+            n = min(CH, steps - c0)
+            self._ehr_factors(ts_all.tensor[c0:c0 + n], n)
+            graphs.run_chunk(b["graphs"], "ehrenfest", lambda n=n: self._ehr_chunk(n), n == CH)
+            ring.tensor[c0:c0 + n].copy_(b["stage"].tensor[:n])
         self._flush_scale()
         self.t = ts[-1]
         r = ring.get()
@@ -474,6 +499,59 @@ extern "C" __global__ void aw_init(%(st)s *psi, int n)
         self.P_average.extend((r[:, 6] / npm).tolist())
         self.hamiltonian_average.extend((r[:, 3] / nx + r[:, 7] / npm).tolist())
         return self
+
+    def _ehr_bufs(self):
+        """Persistent inputs and outputs of the Ehrenfest loop's chunks (fixed addresses, so
+        that a chunk can be captured and replayed)."""
+        b = getattr(self, "_ehr", None)
+        if b is None:
+            N, CH, p = self.X_gridDIM, CHUNK_STEPS, self.plan
+            b = self._ehr = dict(ts=empty((CH,), np.float64), Vt=empty((CH, N), np.float64),
+                                 Kt=empty((CH, N), np.float64), ev=p.new_factors(N, CH), ek=p.new_factors(N, CH),
+                                 stage=empty((CH, 8), np.float64), part=empty((self._wblocks(), 4), np.float64),
+                                 xs=["1.", "X", self.diff_V, self.V], ps=["1.", self.diff_K, "P", self.K],
+                                 graphs={})
+            # compile the observable modules and run the reductions once outside any capture
+            self._obs_module(b["xs"])
+            self._obs_module(b["ps"])
+            self._obs_module([])
+            self.plan.stats(self.wavefunction, out=self._norm_st)
+        return b
+
+    def _ehr_factors(self, ts_dev_slice, n):
+        """EV and EK of the chunk's n steps (one vector each per step; a step applies its EV
+        at both ends, :356-369) from the step times, which are copied on the device."""
+        b, N, dt, p = self._ehr, self.X_gridDIM, self.dt, self.plan
+        b["ts"].tensor[:n].copy_(ts_dev_slice)
+        self._eval_1d(0, self._dev_kX, N, b["ts"], n, 0.0, b["Vt"])
+        self._eval_1d(1, self._dev_kP, N, b["ts"], n, 0.0, b["Kt"])
+        bx = None if self._bx is None else self._bx.ptr
+        bp = None if self._bp is None else self._bp.ptr
+        p.factors(b["ev"], N, n, A=b["Vt"].ptr, ca=-0.5 * dt, mag=bx)
+        p.factors(b["ek"], N, n, A=b["Kt"].ptr, ca=-dt, mag=bp, scale=1.0 / N)
+
+    def _ehr_chunk(self, n):
+        """n steps EV F EK I EV (the first pass applies the pending normalisation), each
+        followed by the Ehrenfest sums of the x- and p-representations into the stage row and
+        the next normalisation scale 1 / (||psi|| sqrt(dX)).  Only launches, from fixed
+        addresses."""
+        b, N, p, psi = self._ehr, self.X_gridDIM, self.plan, self.wavefunction
+        csz = p.fsize
+        for s in range(n):
+            ev, ek = b["ev"].ptr + s * N * csz, b["ek"].ptr + s * N * csz
+            if self._two is None:
+                p.run(psi, 1, 0, FWD, elem_fac=ev, dscale=self._dscale)
+                p.run(psi, 1, 0, INV, elem_fac=ek)
+                p.run(psi, 1, 0, 0, elem_fac=ev)
+            else:
+                self._col(psi, 0, FWD, ctab=ev, twiddle=True, dscale=self._dscale)
+                self._row(psi, FWD, INV, ctab=ek, twiddle=True)
+                self._col(psi, INV, 0, ctab=ev)
+            t_ptr, row = b["ts"].ptr + 8 * s, b["stage"].ptr + 64 * s
+            self._weighted_dev(b["xs"], row, t_ptr, b["part"])
+            self._fft(psi, FWD, out=self._tmp)
+            self._weighted_dev(b["ps"], row + 32, t_ptr, b["part"], psi=self._tmp, p_rep=True)
+            self._norm_scale(np.sqrt(self.dX))
 
     def get_Ehrenfest(self, t):
         """:412-459 (single evaluation on the current state; normalises psi like the reference)."""

@@ -39,7 +39,17 @@ def _rv():
     return g
 
 
-@pytest.mark.parametrize("make,state", [(_wm, "wignerfunction"), (_rv, "rho"), (_wm_merged, "wignerfunction")])
+def _sw():
+    from accuratewigner_amd import SchrodingerWignerCUDA1D
+    g = SchrodingerWignerCUDA1D(X_gridDIM=8192, X_amplitude=60., dt=0.01, t=0., K="0.5 * P * P",
+                                V="-1. / sqrt(X * X + 1.37) + 0.05 * X * sin(0.06 * t)", diff_K="P",
+                                diff_V="X / pow(X * X + 1.37, 1.5) + 0.05 * sin(0.06 * t)")
+    g.set_wavefunction("exp(-X * X)")
+    return g
+
+
+@pytest.mark.parametrize("make,state", [(_wm, "wignerfunction"), (_rv, "rho"), (_wm_merged, "wignerfunction"),
+                                        (_sw, "wavefunction")])
 def test_captured_chunks_equal_eager(make, state, monkeypatch):
     import torch
     names = ("X_average", "P_average", "X_average_RHS", "P_average_RHS", "hamiltonian_average")
