@@ -43,8 +43,12 @@ template <typename R, int N, bool STRIDED> struct PassTune {
     static constexpr int fit(int lpb, int tw) {
         return lpb <= 1 ? 1 : (lds_bytes_dyn(lpb, tw) <= kLdsBudget ? lpb : fit(lpb / 2, tw));
     }
+    // row pad: one slot per 32 positions in FP32; per 16 with FP64 arithmetic (FP64 1024-point
+    // rows, 12-state group launches: 80.3 -> 78.4 us; complex64 states with FP64 arithmetic,
+    // 24 states: 141.3 -> 136.5 us; per 8 or 64: slower)
+    static constexpr int PADR = F32 ? 5 : 4;
     static constexpr int lds_bytes_dyn(int lpb, int tw) {
-        return ((STRIDED ? (N + (N >> 4)) * lpb : lpb * (N + (N >> 5) + 1)) + (tw ? N : 0)) *
+        return ((STRIDED ? (N + (N >> 4)) * lpb : lpb * (N + (N >> PADR) + 1)) + (tw ? N : 0)) *
                int(sizeof(typename Cx<R>::T));
     }
     // FP64 columns of 512 and 1024 points go 4 to a 256-thread block with the twiddles read
@@ -105,7 +109,8 @@ template <typename R, int N, bool STRIDED> struct SmallTune {
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0, typename S = R>
 using PassK = KCfg<R, N, PassTune<R, N, STRIDED>::E, PassTune<R, N, STRIDED>::LPB, STRIDED,
                    PassTune<R, N, STRIDED>::TWM, DA, DB, GEN, 1, PassTune<R, N, STRIDED>::PF,
-                   (PassTune<R, N, STRIDED>::XS && STRIDED) ? 5 : 4, 5, OBS, PassTune<R, N, STRIDED>::XS, S>;
+                   (PassTune<R, N, STRIDED>::XS && STRIDED) ? 5 : 4, PassTune<R, N, STRIDED>::PADR, OBS,
+                   PassTune<R, N, STRIDED>::XS, S>;
 
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0, typename S = R>
 using PassKS = KCfg<R, N, SmallTune<R, N, STRIDED>::E, SmallTune<R, N, STRIDED>::LPB, STRIDED, 1, DA,
