@@ -112,6 +112,8 @@ _SIGNATURES = {
     "aw_plan_create": [ctypes.POINTER(_vp), _i, _i, _i, _i],
     "aw_plan_destroy": [_vp],
     "aw_fft_axis": [_vp, _vp, _vp, _i, _i, _vp],
+    "aw_fft_r2c": [_vp, _vp, _vp, _i, _vp],
+    "aw_fft_c2r": [_vp, _vp, _vp, _i, _vp],
     "aw_pass_run": [_vp, _vp, ctypes.POINTER(aw_pass), _vp],
     "aw_row_obs_blocks": [_vp, ctypes.POINTER(_i)],
     "aw_pass_obs_blocks": [_vp, _i, ctypes.POINTER(_i)],

@@ -22,6 +22,7 @@ struct aw_plan {
     double *diag_work = nullptr;   // cyclic-diagonal partial sums, allocated on first use
     double *wsum_partials = nullptr;    // weighted-sum block partials, allocated on first use
     unsigned *wsum_counters = nullptr;  // their per-item tickets (zero between launches)
+    void *rc_work = nullptr;            // full complex spectrum of the real transforms, on first use
 };
 
 struct aw_module {
@@ -126,6 +127,7 @@ int aw_plan_destroy(aw_plan *p) {
     if (p->diag_work) (void)hipFree(p->diag_work);
     if (p->wsum_partials) (void)hipFree(p->wsum_partials);
     if (p->wsum_counters) (void)hipFree(p->wsum_counters);
+    if (p->rc_work) (void)hipFree(p->rc_work);
     delete p;
     return AW_OK;
 }
@@ -360,6 +362,39 @@ int aw_fft_axis(aw_plan *p, void *data, void *out, int axis, int dir, aw_stream 
     ps.scale = 1.0;
     ps.out = out;
     return aw_pass_run(p, data, &ps, stream);
+}
+
+// Real transforms run the complex FFT of the full length in a plan-owned workspace (one
+// complex array of the plan's size, allocated by the first call, which must not be captured).
+static int real_transform(aw_plan *p, const void *in, void *out, int axis, int fwd, aw_stream stream) {
+    if (!p) return set_error(AW_ERR_INVALID_PLAN, "NULL plan");
+    if (!in || !out) return set_error(AW_ERR_INVALID_VALUE, "NULL data");
+    if (axis != 0 && axis != 1) return set_error(AW_ERR_INVALID_VALUE, "axis must be 0 or 1");
+    if (axis == 0 && p->rows == 1) return set_error(AW_ERR_INVALID_VALUE, "axis-0 transform on a single-row plan");
+    if (p->prec != AW_PREC_F64 && p->prec != AW_PREC_F32)
+        return set_error(AW_ERR_INVALID_TYPE, "real transforms need an F64 or F32 plan");
+    if (!p->rc_work) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        AW_HIP(hipStreamIsCapturing((hipStream_t)stream, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return set_error(AW_ERR_INVALID_VALUE, "the first real transform on a plan must not be captured");
+        AW_HIP(hipMalloc(&p->rc_work, (size_t)p->csize * p->rows * p->cols * p->batch));
+    }
+    const int n = axis ? p->cols : p->rows;
+    const long long nl = axis ? (long long)p->rows * p->batch : p->batch;
+    const hipStream_t st = (hipStream_t)stream;
+    int rc = launch_real_complex(p->prec, fwd ? 0 : 2, in, p->rc_work, nl, n, p->cols, axis, st);
+    if (rc == AW_OK) rc = aw_fft_axis(p, p->rc_work, nullptr, axis, fwd ? AW_FORWARD : AW_INVERSE, stream);
+    if (rc == AW_OK) rc = launch_real_complex(p->prec, fwd ? 1 : 3, p->rc_work, out, nl, n, p->cols, axis, st);
+    return rc;
+}
+
+int aw_fft_r2c(aw_plan *p, const void *in, void *out, int axis, aw_stream stream) {
+    return real_transform(p, in, out, axis, 1, stream);
+}
+
+int aw_fft_c2r(aw_plan *p, const void *in, void *out, int axis, aw_stream stream) {
+    return real_transform(p, in, out, axis, 0, stream);
 }
 
 int aw_make_factors(aw_plan *p, void *out, long long out_stride, int n, int nvec, int mode, const double *A,

@@ -671,6 +671,75 @@ int launch_convert(const void *src, int sp, void *dst, int dp, long long n, hipS
 }
 
 // ---------------------------------------------------------------------------------------
+// real <-> complex transforms: the packing around a complex FFT of the full length
+// ---------------------------------------------------------------------------------------
+// The index helpers address a batch of lines of n points along one axis: axis 1, lines of n
+// contiguous points (nl lines); axis 0, items of [n][cols] (nl items), the lines being their
+// columns.  A half spectrum keeps h = n/2 + 1 points of each line.
+//   mode 0: complex(re[i], 0) for every element (real input widened)
+//   mode 1: out = the first h points of each line of the full spectrum in
+//   mode 2: the full spectrum of each line from its half: k < h ? in[k] : conj(in[n - k])
+//   mode 3: out = Re(in) for every element
+template <typename R>
+__global__ void k_real_complex(int mode, const void *in, void *out, long long nl, int n, int cols, int axis) {
+    using T2 = typename Cx<R>::T;
+    const int h = n / 2 + 1;
+    const long long m = (mode == 1) ? (axis ? nl * h : nl * h * cols) : (axis ? nl * n : nl * n * cols);
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < m;
+         idx += (long long)gridDim.x * blockDim.x) {
+        if (mode == 0) {
+            reinterpret_cast<T2 *>(out)[idx] = mk<T2>(double(reinterpret_cast<const R *>(in)[idx]), 0.0);
+        } else if (mode == 3) {
+            reinterpret_cast<R *>(out)[idx] = reinterpret_cast<const T2 *>(in)[idx].x;
+        } else if (mode == 1) {
+            long long src;
+            if (axis) {
+                const long long l = idx / h;
+                src = l * n + (idx - l * h);
+            } else {
+                const long long t = idx / cols, c = idx - t * cols, i = t / h, k = t - i * h;
+                src = (i * n + k) * cols + c;
+            }
+            reinterpret_cast<T2 *>(out)[idx] = reinterpret_cast<const T2 *>(in)[src];
+        } else {
+            long long l, k, c = 0;
+            if (axis) {
+                l = idx / n;
+                k = idx - l * n;
+            } else {
+                const long long t = idx / cols;
+                c = idx - t * cols;
+                l = t / n;
+                k = t - l * n;
+            }
+            const bool low = k < h;
+            const long long kk = low ? k : n - k;
+            const long long src = axis ? l * h + kk : (l * h + kk) * cols + c;
+            T2 v = reinterpret_cast<const T2 *>(in)[src];
+            if (!low) v.y = -v.y;
+            reinterpret_cast<T2 *>(out)[idx] = v;
+        }
+    }
+}
+
+int launch_real_complex(int prec, int mode, const void *in, void *out, long long nl, int n, int cols, int axis,
+                        hipStream_t st) {
+    const long long m = (long long)(mode == 1 ? n / 2 + 1 : n) * nl * (axis ? 1 : cols);
+    long long blocks = (m + 255) / 256;
+    if (blocks < 1) return AW_OK;
+    if (blocks > 65536) blocks = 65536;
+    if (prec == AW_PREC_F64)
+        hipLaunchKernelGGL(k_real_complex<double>, dim3((unsigned)blocks), dim3(256), 0, st, mode, in, out, nl, n,
+                           cols, axis);
+    else
+        hipLaunchKernelGGL(k_real_complex<float>, dim3((unsigned)blocks), dim3(256), 0, st, mode, in, out, nl, n,
+                           cols, axis);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(AW_ERR_EXEC_FAILED, "k_real_complex: %s", hipGetErrorString(e));
+    return AW_OK;
+}
+
+// ---------------------------------------------------------------------------------------
 // Gram-Schmidt projection of one vector against nphi unit vectors (single workgroup)
 // ---------------------------------------------------------------------------------------
 // block-wide sum of two doubles (all threads get the result); uniform control flow only

@@ -95,5 +95,8 @@ int launch_scale(int prec, void *data, long long n, int batch, const double *dsc
                  hipStream_t st);
 int launch_project_out(int prec, void *psi, const void *phis, int nphi, int n, hipStream_t st);
 int launch_convert(const void *src, int sp, void *dst, int dp, long long n, hipStream_t st);
+// the packing of real <-> complex transforms (modes: see k_real_complex)
+int launch_real_complex(int prec, int mode, const void *in, void *out, long long nl, int n, int cols, int axis,
+                        hipStream_t st);
 
 }  // namespace awig

@@ -5,11 +5,18 @@ cufftCheckStatus (:31-104), cufftPlanMany / cufftDestroy / cufftExecZ2Z (:107-12
 PlanZ2Z (:243-271), Plan_Z2Z_2D_Axis0 / Plan_Z2Z_2D_Axis1 (:274-345) and fft_Z2Z / ifft_Z2Z
 (:233-238): batched, unnormalised complex FFTs, forward = exp(-2 pi i jk/N).  Arrays are
 ``DeviceArray`` (or any object with ``.gpudata`` and ``.shape``); CUFFT_C2C plans (complex64)
-use the FP32 engine.
+use the FP32 engine.  The real transforms: Plan2DAxis0 / Plan2DAxis1 (:160-217) with
+CUFFT_D2Z / CUFFT_Z2D (CUFFT_R2C / CUFFT_C2R in FP32), cu_fft_D2Z / cu_ifft_D2Z and
+cu_fft_Z2D / cu_ifft_Z2D (:129-157), and cufftExecD2Z / cufftExecZ2D / cufftExecR2C /
+cufftExecC2R on plan handles.
 
 cufftPlanMany accepts the layouts the reference builds: rank 1 over contiguous batches
 (istride 1, idist n), rank 1 over the strided columns of a 2D array (idist 1, istride = the
-number of columns = batch), and contiguous rank-2 batches.  Lengths are powers of two from 64
+number of columns = batch), and contiguous rank-2 batches (complex types only).  A real
+transform keeps the n/2 + 1 non-negative frequencies of each line (cuFFT's layout): rows of
+n/2 + 1 complex values for contiguous batches, the first n/2 + 1 rows of the [n, batch]
+array for strided columns.  As in libcufft, D2Z / R2C always transform forward and Z2D / C2R
+inverse: the direction the wrapper passes (cufft.py:134, :138, :151, :156) is ignored.  Lengths are powers of two from 64
 to 4096 (the column count of a strided plan as well); other layouts raise cufftInvalidSize,
 other types cufftInvalidType.  n, inembed and onembed may be host addresses of int32 arrays
 (as the reference passes them) or sequences.
@@ -175,19 +182,27 @@ def _ints(v, count):
     return [int(x) for x in list(v)[:count]]
 
 
+_REAL_TYPES = {CUFFT_D2Z: 1, CUFFT_R2C: 1, CUFFT_Z2D: 0, CUFFT_C2R: 0}   # 1: forward (real input)
+
+
 def cufftPlanMany(rank, n, inembed, istride, idist, onembed, ostride, odist, fft_type, batch):
     """Create a batched FFT plan (cufft.py:107-116); returns the handle as a ctypes.c_uint."""
-    if fft_type not in (CUFFT_Z2Z, CUFFT_C2C):
-        raise cufftInvalidType("only complex-to-complex plans (CUFFT_Z2Z, CUFFT_C2C) are supported")
-    prec = _lib.AW_PREC_F64 if fft_type == CUFFT_Z2Z else _lib.AW_PREC_F32
+    if fft_type not in (CUFFT_Z2Z, CUFFT_C2C) and fft_type not in _REAL_TYPES:
+        raise cufftInvalidType("unknown transform type 0x%x" % int(fft_type))
+    prec = _lib.AW_PREC_F64 if fft_type in (CUFFT_Z2Z, CUFFT_D2Z, CUFFT_Z2D) else _lib.AW_PREC_F32
     dims = _ints(n, rank)
-    if rank not in (1, 2) or batch < 1 or (istride, idist) != (ostride, odist):
+    real = fft_type in _REAL_TYPES
+    # (a real transform's input and output distances differ, n reals against n/2 + 1 complex
+    # values, and cuFFT ignores them without embeddings, as the reference's plans pass them)
+    if rank not in (1, 2) or batch < 1 or (istride, idist if not real else 0) != (ostride, odist if not real else 0):
         raise cufftInvalidSize("unsupported plan: rank %d, batch %d" % (rank, batch))
+    if real and rank != 1:
+        raise cufftInvalidSize("real transforms: rank-1 plans only")
     ie = _ints(inembed, rank)
     oe = _ints(onembed, rank)
     if rank == 2 and ((ie and ie[1:] != dims[1:]) or (oe and oe[1:] != dims[1:])):
         raise cufftInvalidSize("rank-2 plans must be contiguous")
-    if rank == 1 and istride == 1 and idist in (0, dims[0]):
+    if rank == 1 and istride == 1 and (idist in (0, dims[0]) or real):
         plan = _ManyPlan(1, dims[0], batch, prec, (1,))                  # rows of contiguous batches
     elif rank == 1 and idist == 1 and istride == batch:
         plan = _ManyPlan(dims[0], batch, 1, prec, (0,))                  # the columns of [n, batch]
@@ -195,6 +210,7 @@ def cufftPlanMany(rank, n, inembed, istride, idist, onembed, ostride, odist, fft
         plan = _ManyPlan(dims[0], dims[1], batch, prec, (1, 0))          # contiguous 2D transforms
     else:
         raise cufftInvalidSize("unsupported layout: istride %d, idist %d" % (istride, idist))
+    plan.fft_type = fft_type
     h = _NEXT_HANDLE[0]
     _NEXT_HANDLE[0] += 1
     _PLANS[h] = plan
@@ -219,6 +235,8 @@ def cufftExecZ2Z(plan, idata, odata, direction):
     """Execute a complex-to-complex plan on device addresses (cufft.py:224-232): in place when
     idata == odata.  A rank-2 plan transforms along the rows, then along the columns."""
     _, p = _plan_of(plan)
+    if p.fft_type not in (CUFFT_Z2Z, CUFFT_C2C):
+        raise cufftInvalidType("a real-transform plan runs through cufftExecD2Z / cufftExecZ2D")
     if direction not in (CUFFT_FORWARD, CUFFT_INVERSE):
         raise cufftInvalidValue("direction must be CUFFT_FORWARD or CUFFT_INVERSE")
     lib = _lib.load()
@@ -249,3 +267,107 @@ class PlanZ2Z:
             cufftDestroy(self.handle)
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------------------
+# real transforms (cufft.py:129-217)
+# ---------------------------------------------------------------------------------------
+def _exec_real(plan, idata, odata, forward, types):
+    _, p = _plan_of(plan)
+    if p.fft_type not in types:
+        raise cufftInvalidType("plan type 0x%x does not run this transform" % p.fft_type)
+    if int(idata) == int(odata):
+        raise cufftInvalidValue("real transforms run out of place")
+    fn = _lib.load().aw_fft_r2c if forward else _lib.load().aw_fft_c2r
+    cufftCheckStatus(fn(p.handle, ctypes.c_void_p(int(idata)), ctypes.c_void_p(int(odata)), p.axes[0],
+                        ctypes.c_void_p(current_stream())))
+
+
+def cufftExecD2Z(plan, idata, odata, direction=CUFFT_FORWARD):
+    """Double-to-double-complex forward FFT on device addresses (the n/2 + 1 non-negative
+    frequencies of each line); the direction is ignored, as libcufft takes none."""
+    _exec_real(plan, idata, odata, True, (CUFFT_D2Z,))
+
+
+def cufftExecZ2D(plan, idata, odata, direction=CUFFT_INVERSE):
+    """Double-complex-to-double inverse FFT (the half spectrum in, n reals out)."""
+    _exec_real(plan, idata, odata, False, (CUFFT_Z2D,))
+
+
+def cufftExecR2C(plan, idata, odata, direction=CUFFT_FORWARD):
+    """FP32 counterpart of cufftExecD2Z."""
+    _exec_real(plan, idata, odata, True, (CUFFT_R2C,))
+
+
+def cufftExecC2R(plan, idata, odata, direction=CUFFT_INVERSE):
+    """FP32 counterpart of cufftExecZ2D."""
+    _exec_real(plan, idata, odata, False, (CUFFT_C2R,))
+
+
+def _exec_real_arrays(x, y, plan, forward):
+    real_in = plan.fft_type in (CUFFT_D2Z, CUFFT_R2C)
+    if forward != real_in:
+        raise cufftInvalidType("plan type 0x%x does not run this transform" % plan.fft_type)
+    f = cufftExecD2Z if plan.fft_type == CUFFT_D2Z else cufftExecR2C if plan.fft_type == CUFFT_R2C else \
+        cufftExecZ2D if plan.fft_type == CUFFT_Z2D else cufftExecC2R
+    f(plan.handle, int(x.gpudata), int(y.gpudata))
+
+
+def cu_fft_D2Z(x_input_gpu, y_output_gpu, plan):
+    """Real-to-complex FFT through a D2Z plan (cufft.py:133-134)."""
+    _exec_real_arrays(x_input_gpu, y_output_gpu, plan, True)
+
+
+def cu_ifft_D2Z(x_input_gpu, y_output_gpu, plan):
+    """cufft.py:137-138: passes CUFFT_INVERSE to cufftExecD2Z, which takes no direction, so it
+    is the same forward real-to-complex FFT."""
+    _exec_real_arrays(x_input_gpu, y_output_gpu, plan, True)
+
+
+def cu_fft_Z2D(x_input_gpu, y_output_gpu, plan):
+    """cufft.py:150-152: the complex-to-real inverse FFT (cufftExecZ2D takes no direction)."""
+    _exec_real_arrays(x_input_gpu, y_output_gpu, plan, False)
+
+
+def cu_ifft_Z2D(x_input_gpu, y_output_gpu, plan):
+    """Complex-to-real inverse FFT through a Z2D plan (cufft.py:155-157)."""
+    _exec_real_arrays(x_input_gpu, y_output_gpu, plan, False)
+
+
+class _TypedAxisPlan:
+    def __init__(self, shape, cuFFT_type, axis):
+        if len(shape) != 2:
+            raise ValueError('invalid transform dimension')
+        self.shape = tuple(int(s) for s in shape)
+        self.fft_type = cuFFT_type
+        if axis == 0:
+            self.batch = self.shape[1]
+            n = np.array([self.shape[0]], np.int32)
+            stride = self.shape[1]
+            embed = np.array([self.shape[0], stride], np.int32)
+            self.handle = cufftPlanMany(1, n.ctypes.data, embed.ctypes.data, stride, 1, embed.ctypes.data, stride, 1,
+                                        self.fft_type, self.batch)
+        else:
+            self.batch = self.shape[0]
+            n = np.array([self.shape[1]], np.int32)
+            self.handle = cufftPlanMany(1, n.ctypes.data, None, 1, 0, None, 1, 0, self.fft_type, self.batch)
+
+    def __del__(self):
+        try:
+            cufftDestroy(self.handle)
+        except Exception:
+            pass
+
+
+class Plan2DAxis0(_TypedAxisPlan):
+    """Plan of any type over axis 0 of a 2D array (cufft.py:160-185): one transform per column."""
+
+    def __init__(self, shape, cuFFT_type):
+        _TypedAxisPlan.__init__(self, shape, cuFFT_type, 0)
+
+
+class Plan2DAxis1(_TypedAxisPlan):
+    """Plan of any type over axis 1 of a 2D array (cufft.py:188-212): one transform per row."""
+
+    def __init__(self, shape, cuFFT_type):
+        _TypedAxisPlan.__init__(self, shape, cuFFT_type, 1)

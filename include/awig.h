@@ -70,6 +70,20 @@ int aw_plan_destroy(aw_plan *plan); /* cufftDestroy, cufft.py:119-127 */
 int aw_fft_axis(aw_plan *plan, void *data, void *out, int axis, int dir, aw_stream stream);
 
 /*
+ * Real-to-complex forward FFT along one axis (unnormalised): real input with the plan's
+ * shape (double for AW_PREC_F64, float for AW_PREC_F32), complex output keeping the n/2 + 1
+ * non-negative frequencies of each line: [batch][rows][cols/2 + 1] along axis 1,
+ * [batch][rows/2 + 1][cols] along axis 0.  Replaces cufftExecD2Z / cufftExecR2C through
+ * cu_fft_D2Z / cu_ifft_D2Z (cufft.py:129-138).  aw_fft_c2r is the complex-to-real inverse
+ * (the half spectrum in, the n reals of each line out; the imaginary parts of the zero and
+ * Nyquist frequencies are ignored), replacing cufftExecZ2D / cufftExecC2R through
+ * cu_fft_Z2D / cu_ifft_Z2D (cufft.py:146-157).  Out of place.  The first call on a plan
+ * allocates a complex workspace of the plan's size and must not be captured.
+ */
+int aw_fft_r2c(aw_plan *plan, const void *in, void *out, int axis, aw_stream stream);
+int aw_fft_c2r(aw_plan *plan, const void *in, void *out, int axis, aw_stream stream);
+
+/*
  * One fused line pass over every line of the chosen axis of every batch item:
  *     load -> [sign_in] -> [FFT dir_a] -> multiply by factor -> [FFT dir_b] -> [sign_out]
  *          -> store (or store the diagonal element only)

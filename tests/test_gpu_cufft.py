@@ -78,3 +78,59 @@ def test_fft_axis_matches_hipfft(n, prec):
                     ref = torch.fft.ifft(a, dim=axis + 1, norm="forward")
                 err = (torch.linalg.vector_norm(x.tensor - ref) / torch.linalg.vector_norm(ref)).item()
                 assert err < tol, (n, batch, axis, d, err)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("n", [64, 256, 1024, 4096])
+def test_cufft_real_transforms(n, prec):
+    """Plan2DAxis0 / Plan2DAxis1 with the real types (cufft.py:129-217): cu_fft_D2Z keeps the
+    n/2 + 1 non-negative frequencies of each line (numpy rfft), cu_ifft_D2Z is the same forward
+    transform (libcufft's D2Z takes no direction), and cu_fft_Z2D / cu_ifft_Z2D give the n
+    reals of the unnormalised inverse (numpy irfft times n); R2C / C2R in FP32; the handle
+    functions cufftExecD2Z / cufftExecZ2D directly."""
+    from accuratewigner_amd import cufft
+    from accuratewigner_amd.device import empty, to_device
+    fwd_t, inv_t = (cufft.CUFFT_D2Z, cufft.CUFFT_Z2D) if prec == "fp64" else (cufft.CUFFT_R2C, cufft.CUFFT_C2R)
+    rdt, cdt = (np.float64, np.complex128) if prec == "fp64" else (np.float32, np.complex64)
+    tol = 1e-13 if prec == "fp64" else 2e-6
+    rng = np.random.default_rng(n)
+    other = 128          # (a strided column plan's column count is a power of two as well)
+    h = n // 2 + 1
+    for axis, shape in ((1, (other, n)), (0, (n, other))):
+        a = rng.standard_normal(shape)
+        plan_cls = cufft.Plan2DAxis1 if axis == 1 else cufft.Plan2DAxis0
+        pf, pi = plan_cls(shape, fwd_t), plan_cls(shape, inv_t)
+        x = to_device(a.astype(rdt))
+        hs = (other, h) if axis == 1 else (h, other)
+        y = empty(hs, cdt)
+        ref = np.fft.rfft(a, axis=axis)
+        cufft.cu_fft_D2Z(x, y, pf)
+        assert rel(y.get(), ref) < tol, (axis, "fwd")
+        y2 = empty(hs, cdt)
+        cufft.cu_ifft_D2Z(x, y2, pf)
+        assert np.array_equal(y.get(), y2.get())
+        # inverse from a Hermitian half spectrum whose zero and Nyquist terms carry imaginary
+        # parts, which the complex-to-real transform ignores
+        X = ref + 0.3j * (np.arange(h) == 0).reshape((1, h) if axis == 1 else (h, 1))
+        z = to_device(X.astype(cdt))
+        r = empty(shape, rdt)
+        cufft.cu_ifft_Z2D(z, r, pi)
+        back = np.fft.irfft(ref, n=n, axis=axis) * n
+        assert rel(r.get(), back) < tol, (axis, "inv")
+        r2 = empty(shape, rdt)
+        cufft.cu_fft_Z2D(z, r2, pi)
+        assert np.array_equal(r.get(), r2.get())
+        assert rel(r.get() / n, a) < tol                 # round trip
+        with pytest.raises(cufft.cufftInvalidType):
+            cufft.cu_fft_Z2D(x, y, pf)                   # a forward plan does not run the inverse
+    # the handle API on contiguous batches, out of place only
+    hp = cufft.cufftPlanMany(1, np.array([n], np.int32).ctypes.data, None, 1, 0, None, 1, 0, fwd_t, 5)
+    b = rng.standard_normal((5, n))
+    xb, yb = to_device(b.astype(rdt)), empty((5, h), cdt)
+    (cufft.cufftExecD2Z if prec == "fp64" else cufft.cufftExecR2C)(hp, xb.gpudata, yb.gpudata)
+    assert rel(yb.get(), np.fft.rfft(b, axis=1)) < tol
+    with pytest.raises(cufft.cufftInvalidType):
+        cufft.cufftExecZ2Z(hp, xb.gpudata, xb.gpudata, cufft.CUFFT_FORWARD)
+    with pytest.raises(cufft.cufftInvalidValue):
+        (cufft.cufftExecD2Z if prec == "fp64" else cufft.cufftExecR2C)(hp, xb.gpudata, xb.gpudata)
+    cufft.cufftDestroy(hp)

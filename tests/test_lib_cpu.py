@@ -118,12 +118,15 @@ def test_entry_points_reject_null_and_bad_arguments_before_touching_the_gpu():
         assert _lib.load().aw_last_error()          # a message is left for the caller
 
 
-def test_cufft_plan_many_rejects_real_types_without_a_gpu_call():
+def test_cufft_plan_many_argument_checks_without_a_gpu_call():
+    """Unknown transform types and rank-2 real transforms are refused before any GPU call."""
     from accuratewigner_amd import cufft
     import numpy as np
+    with pytest.raises(cufft.cufftInvalidType):
+        cufft.cufftPlanMany(1, np.array([64], np.int32).ctypes.data, None, 1, 0, None, 1, 0, 0x99, 1)
     for t in (cufft.CUFFT_R2C, cufft.CUFFT_C2R, cufft.CUFFT_D2Z, cufft.CUFFT_Z2D):
-        with __import__("pytest").raises(cufft.cufftInvalidType):
-            cufft.cufftPlanMany(1, np.array([64], np.int32).ctypes.data, None, 1, 0, None, 1, 0, t, 1)
+        with pytest.raises(cufft.cufftInvalidSize):
+            cufft.cufftPlanMany(2, np.array([64, 64], np.int32).ctypes.data, None, 1, 0, None, 1, 0, t, 1)
 
 
 def test_wm_physics_source_with_time_dependent_masks_compiles():
