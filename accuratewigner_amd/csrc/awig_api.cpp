@@ -23,6 +23,9 @@ struct aw_plan {
     double *wsum_partials = nullptr;    // weighted-sum block partials, allocated on first use
     unsigned *wsum_counters = nullptr;  // their per-item tickets (zero between launches)
     void *rc_work = nullptr;            // full complex spectrum of the real transforms, on first use
+    int *q_scales = nullptr;            // complex64 states with FP64 arithmetic: the scale words of
+                                        // the line fixed point of step sequences, [batch][rows]
+                                        // (row passes) then [batch][cols] (column passes)
 };
 
 struct aw_module {
@@ -111,6 +114,10 @@ int aw_plan_create(aw_plan **plan, int rows, int cols, int batch, int prec) {
         hipError_t e = hipMalloc((void **)&p->partials, sizeof(double) * 8 * (size_t)nb * batch);
         if (e != hipSuccess) rc = set_error(AW_ERR_ALLOC_FAILED, "partials: %s", hipGetErrorString(e));
     }
+    if (rc == AW_OK && prec == AW_PREC_C64 && rows > 1) {
+        hipError_t e = hipMalloc((void **)&p->q_scales, sizeof(int) * (size_t)(rows + cols) * batch);
+        if (e != hipSuccess) rc = set_error(AW_ERR_ALLOC_FAILED, "scale words: %s", hipGetErrorString(e));
+    }
     if (rc != AW_OK) {
         aw_plan_destroy(p);
         return rc;
@@ -128,13 +135,24 @@ int aw_plan_destroy(aw_plan *p) {
     if (p->wsum_partials) (void)hipFree(p->wsum_partials);
     if (p->wsum_counters) (void)hipFree(p->wsum_counters);
     if (p->rc_work) (void)hipFree(p->rc_work);
+    if (p->q_scales) (void)hipFree(p->q_scales);
     delete p;
     return AW_OK;
 }
 
+// The line fixed point of a step pass (step sequences of complex64 states with FP64
+// arithmetic): q bit 0 reads it with the scale words q_in, bit 1 writes it and its scale
+// words to q_out.
+struct StepQ {
+    int q = 0;
+    const int *q_in = nullptr;
+    int *q_out = nullptr;
+};
+
 // One pass over the first `nbatch` items at `data` (the plan's batch for aw_pass_run; a group
 // of items for the step sequences).
-static int pass_run_n(aw_plan *p, void *data, const aw_pass *ps, int nbatch, aw_stream stream) {
+static int pass_run_n(aw_plan *p, void *data, const aw_pass *ps, int nbatch, aw_stream stream,
+                      const StepQ &sq = StepQ()) {
     if (!data || !ps) return set_error(AW_ERR_INVALID_VALUE, "NULL data or pass");
     if (ps->axis != 0 && ps->axis != 1) return set_error(AW_ERR_INVALID_VALUE, "axis must be 0 or 1");
     if (ps->axis == 0 && p->rows == 1)
@@ -236,6 +254,9 @@ static int pass_run_n(aw_plan *p, void *data, const aw_pass *ps, int nbatch, aw_
     a.has_factor = (ps->line_fac || ps->elem_fac || ps->mask || ps->ctab || ps->chirp != 0.0 ||
                     ps->scale != 1.0 || ps->dscale) ? 1 : 0;
     a.tw = strided ? p->tw_rows : p->tw_cols;
+    a.q = sq.q;
+    a.q_in = sq.q_in;
+    a.q_out = sq.q_out;
     return launch_pass(p->prec, n, strided, a, (hipStream_t)stream);
 }
 
@@ -324,6 +345,14 @@ int aw_step_sequence(aw_plan *p, void *data, const aw_steps *st, aw_stream strea
             return m ? static_cast<const char *>(m) + b0 * st->mask_bstride * rsz : nullptr;
         };
         const void *m_oc = moff(st->mask_open_close), *m_mid = moff(st->mask_mid), *m_row = moff(st->mask_row);
+        // complex64 states with FP64 arithmetic: between the opening and the closing pass the
+        // state is held in line fixed point (int32 pairs, a power-of-two scale word per line:
+        // 2^-30 of the line's largest value instead of 2^-24 of each value), so the sequence
+        // rounds to complex64 once, at its end, instead of on every pass
+        int *qr = p->q_scales ? p->q_scales + (long long)b0 * p->rows : nullptr;
+        int *qc = p->q_scales ? p->q_scales + (long long)p->batch * p->rows + (long long)b0 * p->cols : nullptr;
+        const StepQ q_open = qr ? StepQ{2, nullptr, qc} : StepQ(), q_row = qr ? StepQ{3, qc, qr} : StepQ();
+        const StepQ q_mid = qr ? StepQ{3, qr, qc} : StepQ(), q_close = qr ? StepQ{1, qr, nullptr} : StepQ();
         aw_pass ps{};
         ps.scale = 1.0;
         ps.mask_bstride = st->mask_bstride;
@@ -333,21 +362,21 @@ int aw_step_sequence(aw_plan *p, void *data, const aw_steps *st, aw_stream strea
         ps.elem_fac = gce; ps.elem_fac_bstride = st->col_elem_bstride;
         ps.mask = m_oc;
         ps.dscale = st->dscale ? st->dscale + 2 * b0 : nullptr;
-        rc = pass_run_n(p, d, &ps, nb, stream);
+        rc = pass_run_n(p, d, &ps, nb, stream, q_open);
         ps.dscale = nullptr;
         for (int s = 0; rc == AW_OK && s < n; ++s) {
             ps.axis = 1; ps.dir_a = AW_FORWARD; ps.dir_b = AW_INVERSE;
             ps.line_fac = grl + rvec(s) * p->rows * fsz; ps.line_fac_bstride = st->row_line_bstride;
             ps.elem_fac = gre + rvec(s) * p->cols * fsz; ps.elem_fac_bstride = st->row_elem_bstride;
             ps.mask = m_row;
-            rc = pass_run_n(p, d, &ps, nb, stream);
+            rc = pass_run_n(p, d, &ps, nb, stream, q_row);
             if (rc != AW_OK) break;
             const bool last = s == n - 1;
             ps.axis = 0; ps.dir_a = AW_FORWARD; ps.dir_b = last ? 0 : AW_INVERSE;
             ps.line_fac = gcl + cvec(s + 1) * p->cols * fsz; ps.line_fac_bstride = st->col_line_bstride;
             ps.elem_fac = gce + cvec(s + 1) * p->rows * fsz; ps.elem_fac_bstride = st->col_elem_bstride;
             ps.mask = last ? m_oc : m_mid;
-            rc = pass_run_n(p, d, &ps, nb, stream);
+            rc = pass_run_n(p, d, &ps, nb, stream, last ? q_close : q_mid);
         }
     }
     return rc;

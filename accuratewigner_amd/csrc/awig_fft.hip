@@ -88,6 +88,7 @@ int launch_pass(int prec, int n, bool strided, const PassArgs &a, hipStream_t st
         }
     }
     if (prec == AW_PREC_C64) return launch_pass_c64(n, strided, kind, a, st);
+    if (a.q) return set_error(AW_ERR_INVALID_VALUE, "line fixed point needs complex64 states with FP64 arithmetic");
     return prec == AW_PREC_F64 ? launch_pass_f64(n, strided, kind, a, st)
                                : launch_pass_f32(n, strided, kind, a, st);
 }

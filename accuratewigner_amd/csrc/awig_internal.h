@@ -47,6 +47,11 @@ struct PassArgs {
     const void *obs_lf, *obs_ef;
     int obs_nd;             // obs_mode 2: diagonal weights (obs_n2w rows)
     void *aux_out;          // general passes: the loaded input times dscale[b], stored here
+    int q;                  // step passes of complex64 states with FP64 arithmetic: bit 0 the
+                            // input, bit 1 the output in line fixed point (int32 pairs, one
+                            // power-of-two scale word per line)
+    const int *q_in;        // [batch][N] scale words of the input (lines of the other axis)
+    int *q_out;             // [batch][nlines] scale words of the output
 };
 
 // obs_mode 2: at most this many cyclic (momentum) and diagonal (coordinate) weight rows

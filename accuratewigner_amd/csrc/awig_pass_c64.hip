@@ -4,6 +4,7 @@
 
 namespace awig {
 int launch_pass_c64(int n, bool strided, int kind, const PassArgs &a, hipStream_t st) {
+    if (a.q) return launch_step_q<double, float>(n, strided, kind, a, st);
     return launch_pass_prec<double, float>(n, strided, kind, a, st);
 }
 }  // namespace awig

@@ -106,15 +106,15 @@ template <typename R, int N, bool STRIDED> struct SmallTune {
 // Column exchanges in real and imaginary halves pad one row per 32 positions (PADC 5)
 // instead of 16: +2 % on 1024^2 x 96 mixed (8: +0.7 %, 64 and 128: slower); the row pad
 // stays at 32 positions (16 is 3 % slower, 64 no faster).
-template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0, typename S = R>
+template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0, typename S = R, int Q = 0>
 using PassK = KCfg<R, N, PassTune<R, N, STRIDED>::E, PassTune<R, N, STRIDED>::LPB, STRIDED,
                    PassTune<R, N, STRIDED>::TWM, DA, DB, GEN, 1, PassTune<R, N, STRIDED>::PF,
                    (PassTune<R, N, STRIDED>::XS && STRIDED) ? 5 : 4, PassTune<R, N, STRIDED>::PADR, OBS,
-                   PassTune<R, N, STRIDED>::XS, S>;
+                   PassTune<R, N, STRIDED>::XS, S, Q>;
 
-template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0, typename S = R>
+template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0, typename S = R, int Q = 0>
 using PassKS = KCfg<R, N, SmallTune<R, N, STRIDED>::E, SmallTune<R, N, STRIDED>::LPB, STRIDED, 1, DA,
-                    DB, GEN, 1, (GEN || OBS >= 3) ? 0 : SmallTune<R, N, STRIDED>::PF, 4, 5, OBS, 0, S>;
+                    DB, GEN, 1, (GEN || OBS >= 3) ? 0 : SmallTune<R, N, STRIDED>::PF, 4, 5, OBS, 0, S, Q>;
 
 template <typename C>
 int launch_cfg(const PassArgs &a, hipStream_t st) {
@@ -236,6 +236,42 @@ template <typename R> int obs_lpb(int n, bool strided, long long total_lines) {
     switch (n) {
         AW_CASE(64) AW_CASE(128) AW_CASE(256) AW_CASE(512) AW_CASE(1024) AW_CASE(2048) AW_CASE(4096)
     default: return 0;
+    }
+#undef AW_CASE
+}
+
+// The step passes of a sequence that keeps its state in line fixed point (complex64 states
+// with FP64 arithmetic, a.q): the opening column pass [factor, inverse] writes it (q = 2),
+// the passes [forward, factor, inverse] read and write it (q = 3) and the closing column pass
+// [forward, factor] reads it (q = 1).  Same tiling as the complex64 step passes.
+template <typename R, int N, bool STRIDED, typename S>
+int launch_step_q_n(int kind, const PassArgs &a, hipStream_t st) {
+    const int want = kind == 1 ? 2 : (kind == 2 ? 3 : (kind == 3 ? 1 : -1));
+    if (a.q != want || (!STRIDED && kind != 2) || !a.q_in != !(a.q & 1) || !a.q_out != !(a.q & 2))
+        return set_error(AW_ERR_INVALID_VALUE, "line fixed point: no step pass of kind %d with q = %d here", kind, a.q);
+    if constexpr (N <= 2048) {
+        if (a.total_lines <= small_lines_max<R, N>()) {
+            if constexpr (STRIDED) {
+                if (kind == 1) return launch_cfg<PassKS<R, N, true, 0, 1, 0, 0, S, 2>>(a, st);
+                if (kind == 3) return launch_cfg<PassKS<R, N, true, -1, 0, 0, 0, S, 1>>(a, st);
+            }
+            return launch_cfg<PassKS<R, N, STRIDED, -1, 1, 0, 0, S, 3>>(a, st);
+        }
+    }
+    if constexpr (STRIDED) {
+        if (kind == 1) return launch_cfg<PassK<R, N, true, 0, 1, 0, 0, S, 2>>(a, st);
+        if (kind == 3) return launch_cfg<PassK<R, N, true, -1, 0, 0, 0, S, 1>>(a, st);
+    }
+    return launch_cfg<PassK<R, N, STRIDED, -1, 1, 0, 0, S, 3>>(a, st);
+}
+
+template <typename R, typename S>
+int launch_step_q(int n, bool strided, int kind, const PassArgs &a, hipStream_t st) {
+#define AW_CASE(NN) \
+    case NN: return strided ? launch_step_q_n<R, NN, true, S>(kind, a, st) : launch_step_q_n<R, NN, false, S>(kind, a, st);
+    switch (n) {
+        AW_CASE(64) AW_CASE(128) AW_CASE(256) AW_CASE(512) AW_CASE(1024) AW_CASE(2048) AW_CASE(4096)
+    default: return set_error(AW_ERR_INVALID_SIZE, "unsupported line length %d (64..4096, power of two)", n);
     }
 #undef AW_CASE
 }

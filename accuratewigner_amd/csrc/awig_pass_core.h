@@ -56,6 +56,32 @@ template <typename D2, typename S2> __device__ __forceinline__ D2 cvt2(S2 a) {
 }
 
 // ---------------------------------------------------------------------------------------
+// Line fixed point: the FP64 values of a line stored as int32 pairs m with one power-of-two
+// scale per line, x = m 2^e (the line's scale word is e).
+//   decode  x = ldexp(double(m), e)                                       (exact)
+//   encode  m = low word of (x + 1.5 2^(52+e))   (the add rounds x / 2^e to nearest, ties even:
+//           x + 1.5 2^(52+e) lies in [2^(52+e), 2^(53+e)), whose spacing is 2^e)
+// e follows from the line's largest |re|, |im| M (M < 2^(e+30), so |m| <= 2^30): a component
+// is off by at most 2^-30 M, against 2^-24 |x| for complex64, so a line's L2 error is smaller
+// unless its largest value exceeds about 2^5 times its root mean square.  The lines written
+// here are FFT outputs, where one Inf or NaN input makes every output non-finite; such a line
+// is stored as m = 1 with e = kBfpInf, which decodes to Inf and turns the next FFT to NaN.
+// ---------------------------------------------------------------------------------------
+constexpr int kBfpInf = 2000;
+__device__ __forceinline__ double2 bfp_decode(int2 m, int e) {
+    return make_double2(ldexp(double(m.x), e), ldexp(double(m.y), e));
+}
+// the scale word of a line whose largest |re|, |im| is mx: mx in [2^(ex-1023), 2^(ex-1022))
+// gives e = ex - 1052 (ex: biased exponent; 0 for zero and subnormals, e = -1052)
+__device__ __forceinline__ int bfp_scale(double mx) {
+    const int ex = int((unsigned(__double2hiint(mx)) & 0x7fffffffu) >> 20);
+    return ex > 2023 ? kBfpInf : ex - 1052;   // Inf, NaN or above 2^1001
+}
+__device__ __forceinline__ int2 bfp_encode(double2 v, double c) {
+    return make_int2(__double2loint(v.x + c), __double2loint(v.y + c));
+}
+
+// ---------------------------------------------------------------------------------------
 // compile-time roots of unity: cos/sin(2 pi k / n) by exact octant reduction of k/n and a
 // Taylor series on [0, pi/4] (evaluated by the compiler, never at run time)
 // ---------------------------------------------------------------------------------------
@@ -335,10 +361,14 @@ __device__ __forceinline__ void fft_line(int dir, T2 *v, T2 *L, int l, int u, co
 // which keeps their code to two FFT bodies.
 template <typename R_, int N_, int E_, int LPB_, bool STRIDED_, int TWM_, int DA_ = 2, int DB_ = 2,
           int GEN_ = 1, int MINW_ = 1, int PF_ = 0, int PADC_ = 4, int PADR_ = 5, int OBS_ = 0, int XS_ = 0,
-          typename S_ = R_>
+          typename S_ = R_, int Q_ = 0>
 struct KCfg {
     using R = R_;   // arithmetic, twiddles, factors and masks
     using S = S_;   // the state in HBM (converted on load and store)
+    // line fixed-point storage (FP64 arithmetic on 8-byte states only): bit 0 the input,
+    // bit 1 the output is int32 pairs scaled by a power of two per line (see bfp_decode)
+    static constexpr int Q = Q_;
+    static_assert(Q == 0 || (sizeof(R_) == 8 && sizeof(S_) == 4), "line fixed point: FP64 arithmetic, 8-byte states");
     static constexpr int N = N_, E = E_, LPB = LPB_, TWM = TWM_, DA = DA_, DB = DB_, GEN = GEN_;
     static constexpr int OBS = OBS_;     // weighted-sum block partials of the output (1) or of the
                                          // values right after the factor multiply (2); von Neumann
@@ -605,6 +635,9 @@ void k_line_pass(PassArgs a) {
     constexpr int N = C::N, E = C::E, T = C::T, LPB = C::LPB;
     constexpr bool STRIDED = C::STRIDED;
     __shared__ T2 smem[C::LDS_ELEMS];
+    // line maxima (high words) of a fixed-point store whose lines span several waves
+    __shared__ unsigned q_lmax[(C::Q & 2) && (C::STRIDED || C::T > 64) ? LPB : 1];
+    static_assert(!(C::Q & 2) || C::DA != 0 || C::DB != 0, "the FFT's barriers order q_lmax's reset");
 
     const int tid = threadIdx.x;
     int l_loc, u;
@@ -625,6 +658,9 @@ void k_line_pass(PassArgs a) {
     const int line = int(gline & (nl - 1));
     T2 *L = smem;
     const T2 *tw = reinterpret_cast<const T2 *>(a.tw);
+    if constexpr ((C::Q & 2) && (STRIDED || T > 64)) {
+        if (tid < LPB) q_lmax[tid] = 0u;
+    }
     if constexpr (C::TWM != 0) {
         // twiddle table into LDS; the first exchange barrier of any FFT precedes its first use
         T2 *tl = smem + C::XCH_ELEMS;
@@ -674,8 +710,17 @@ void k_line_pass(PassArgs a) {
     using S2 = typename Cx<typename C::S>::T;
     const S2 *src = reinterpret_cast<const S2 *>(a.in) + item + off0;
     T2 v[E];
+    if constexpr (C::Q & 1) {
+        // element e of this line lies on line e of the previous pass (the other axis), whose
+        // scale word that pass stored
+        const int2 *qsrc = reinterpret_cast<const int2 *>(a.in) + item + off0;
+        const int *qh = a.q_in + b * N + u;
 #pragma unroll
-    for (int k = 0; k < E; ++k) v[k] = cvt2<T2>(src[k * kstep]);
+        for (int k = 0; k < E; ++k) v[k] = bfp_decode(qsrc[k * kstep], qh[T * k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < E; ++k) v[k] = cvt2<T2>(src[k * kstep]);
+    }
     constexpr int NPRE = obs_prefetch<C>();
     double2 wpre[NPRE > 0 ? NPRE : 1][E];
     if constexpr (NPRE > 0) {
@@ -808,8 +853,38 @@ void k_line_pass(PassArgs a) {
     if constexpr (C::OBS == 1) obs_block_partials<C>(a, v, active, off0, kstep, blk, nl, tid, wpre);
     if constexpr (C::OBS == 3) store_block_partials<C>(cacc, 1 + a.obs_nd + a.obs_nw, a, blk, nl, tid);
     if constexpr (C::OBS == 4) store_block_partials<C>(cacc, a.obs_nw, a, blk, nl, tid);
+    int q_e = 0;
+    if constexpr (C::Q & 2) {
+        // the line's largest |re|, |im| (a NaN line is NaN in every element), as the high word
+        // of its FP64 pattern, which orders non-negative values
+        double mx = fabs(v[0].x);
+#pragma unroll
+        for (int k = 0; k < E; ++k) mx = fmax(fmax(mx, fabs(v[k].x)), fabs(v[k].y));
+        unsigned key = unsigned(__double2hiint(mx)) & 0x7fffffffu;
+        if constexpr (!STRIDED && T <= 64) {
+            // the line's T lanes are adjacent in one wave
+#pragma unroll
+            for (int o = T / 2; o >= 1; o >>= 1) key = max(key, unsigned(__shfl_xor(int(key), o)));
+        } else {
+            atomicMax(&q_lmax[l_loc], key);
+            __syncthreads();
+            key = q_lmax[l_loc];
+        }
+        q_e = bfp_scale(__hiloint2double(int(key), 0));
+        if (active && u == 0) a.q_out[b * nl + line] = q_e;
+    }
     if (!active) return;
-    if (!C::GEN || a.out_mode == 0) {
+    if constexpr (C::Q & 2) {
+        int2 *dst = reinterpret_cast<int2 *>(a.out) + item + off0;
+        if (q_e != kBfpInf) {
+            const double c = ldexp(1.5, 52 + q_e);
+#pragma unroll
+            for (int k = 0; k < E; ++k) dst[k * kstep] = bfp_encode(v[k], c);
+        } else {
+#pragma unroll
+            for (int k = 0; k < E; ++k) dst[k * kstep] = make_int2(1, 1);
+        }
+    } else if (!C::GEN || a.out_mode == 0) {
         S2 *dst = reinterpret_cast<S2 *>(a.out) + item + off0;
 #pragma unroll
         for (int k = 0; k < E; ++k) dst[k * kstep] = cvt2<S2>(v[k]);

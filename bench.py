@@ -430,7 +430,10 @@ def main():
                          "roofline_frac": t["achieved"] / HBM_PEAK_GBPS,
                          "speedup_vs_fp64": t["value"] / fp64_line["value"] if fp64_line else None,
                          "rel_l2_err_vs_fp64": errs.get(name),
-                         "arithmetic": "FP32 butterflies and factors" if name == "mixed" else "FP64 butterflies and factors"}
+                         "error_bound_met": (errs[name] <= 1e-6) if errs.get(name) is not None else None,
+                         "arithmetic": "FP32 butterflies and factors" if name == "mixed" else
+                                       "FP64 butterflies and factors; line fixed point (int32, one power-of-two "
+                                       "scale per line) between the passes of a step sequence"}
                   for name, t in tiers.items()},
         "roofline": None if not h else {
             "bound": "hbm", "achieved": h["achieved"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",

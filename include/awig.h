@@ -185,6 +185,9 @@ int aw_row_obs_blocks(const aw_plan *plan, int *nblk);
  * A batch larger than 192 MiB with n >= 2 is stepped in groups of items of at most that
  * size, each group through all 2n + 1 passes, so that it stays in the MALL between passes;
  * the results are bit-identical to stepping the whole batch per pass.
+ * AW_PREC_C64 plans hold the state between the opening and the closing pass as int32 pairs
+ * with one power-of-two scale per line (rounding to 2^-30 of the line's largest component
+ * instead of 2^-24 of each value); `data` is complex64 on entry and on return.
  */
 typedef struct aw_steps {
     int nsteps;

@@ -238,9 +238,10 @@ def test_wm_mixed_precision_error_budget():
 
 
 def test_wm_c64_error_below_mixed():
-    """complex64 states with FP64 arithmetic: only the stored state is rounded, so after 200
-    steps the error against the oracle stays near sqrt(400) roundings of 2^-24 (measured
-    ~1e-6), a few times below the FP32-arithmetic tier's."""
+    """complex64 states with FP64 arithmetic: only the stored state is rounded, in line fixed
+    point (2^-30 of a line's largest value) inside the 200-step sequence and to complex64 at
+    its end, so the error against the oracle stays near 1e-7 (4e-7 bound here; complex64 on
+    every pass measured ~1e-6), far below the FP32-arithmetic tier's."""
     errs = {}
     for prec in ("c64", "mixed"):
         g, o = _pair(precision=prec, V="0.5 * X * X + 0.05 * X * X * X * X",
@@ -250,8 +251,8 @@ def test_wm_c64_error_below_mixed():
         W = g.propagate(200).get()
         assert W.dtype == np.complex64
         errs[prec] = rel(W.astype(np.complex128), o.propagate(200))
-    assert errs["c64"] < 1.5e-6, errs
-    assert errs["c64"] < 0.5 * errs["mixed"], errs
+    assert errs["c64"] < 4e-7, errs
+    assert errs["c64"] < 0.2 * errs["mixed"], errs
 
 
 def test_wm_c64_throughput_configuration_matches_fp64():
