@@ -62,14 +62,34 @@ static void host_root(long long t, long long n, double &c, double &s) {
     s = -ss;  // forward root exp(-i theta)
 }
 
+// The flat table exp(-2 pi i t / n), t < n, followed by the stage-major tables of the
+// Stockham FFTs with E = 4, 8, 16 and 32 elements per thread (n entries each): after the
+// first stage (radix E), the stage of span NS and radix RAD = min(E, n / NS) owns entries
+// [NS - E, NS RAD - E), entry NS - E + k (RAD - 1) + r - 1 = exp(-2 pi i r k / (NS RAD)).
 static int make_twiddles(int n, int prec, void **out) {
-    std::vector<double> h(2 * (size_t)n);
+    std::vector<double> h(2 * (size_t)n * 5, 0.0);
     for (int t = 0; t < n; ++t) host_root(t, n, h[2 * t], h[2 * t + 1]);
+    for (int ei = 1; ei <= 4; ++ei) {
+        const int E = 2 << ei;
+        if (E >= n) continue;
+        double *tab = h.data() + 2 * (size_t)n * ei;
+        for (int ns = E; ns < n;) {
+            const int rad = (n / ns < E) ? n / ns : E;
+            for (int k = 0; k < ns; ++k)
+                for (int r = 1; r < rad; ++r) {
+                    const size_t q = (size_t)(ns - E + k * (rad - 1) + r - 1);
+                    const size_t t = (size_t)(r * k) * (n / (ns * rad));
+                    tab[2 * q] = h[2 * t];
+                    tab[2 * q + 1] = h[2 * t + 1];
+                }
+            ns *= rad;
+        }
+    }
     size_t bytes;
     std::vector<float> hf;
     const void *src = h.data();
     if (prec == AW_PREC_F32) {
-        hf.resize(2 * (size_t)n);
+        hf.resize(h.size());
         for (size_t i = 0; i < hf.size(); ++i) hf[i] = (float)h[i];
         src = hf.data();
         bytes = hf.size() * sizeof(float);

@@ -57,8 +57,11 @@ template <typename R, int N, bool STRIDED> struct PassTune {
     // grid-point steps/s on 1024^2 x 48 (8 columns with the split exchange: 7.2e10; 2 columns:
     // 7.1e10; 4 columns with both: no faster); at 512 points +2 % on 512^2 x 192, equal on x 48
     static constexpr bool C64 = !F32 && STRIDED && E == 16 && N >= 512 && N <= 1024;
-    // keep LDS twiddles only if they do not cost lines per block
-    static constexpr int TWM = C64 ? 0 : ((TW_PREF && fit(WANT, 1) == fit(WANT, 0)) ? 1 : 0);
+    // keep LDS twiddles only if they do not cost lines per block; FP64 rows read them
+    // stage-major (TWM 2: the flat table's radix-16 reads conflict in the LDS banks): FP64
+    // 1024-point rows 78.4 -> 75.0 us per 12-state launch, complex64 states with FP64
+    // arithmetic 144.5 -> 128.2 us per 24 states (FP32 rows: 76.3 -> 76.0 us, kept flat)
+    static constexpr int TWM = C64 ? 0 : ((TW_PREF && fit(WANT, 1) == fit(WANT, 0)) ? (R64X ? 2 : 1) : 0);
     static constexpr int LPB = C64 ? 4 : fit(WANT, TWM);
     // FP32 columns of 2048 points exchange in halves as well (86 -> 51 KB of LDS, 2 -> 3
     // blocks per CU): 1.00e11 -> 1.17e11 grid-point steps/s on 2048^2 x 24.  At 4096 points

@@ -272,7 +272,9 @@ template <bool STRIDED, int N, int LPB, int PADC = 4, int PADR = 5> struct LdsLa
 // TWM 0/1: tw is the flat table exp(-2 pi i t / N) (global or LDS).  TWM 2: tw is the
 // stage-major LDS table: for the stage of span NS and radix RAD, entry (NS - E) + k*(RAD-1)
 // + (r-1) holds exp(-2 pi i r k / (NS RAD)), so a butterfly reads its RAD-1 twiddles at
-// consecutive addresses from one base.
+// consecutive addresses from one base.  In the flat table the lanes of a radix-16 stage read
+// entries r k N / (NS RAD) (k = lane mod NS), which for FP64 fall into one or two LDS banks
+// (8- to 16-way conflicts); the stage-major reads of 8 lanes cover distinct banks.
 // XS 1: the exchange moves the real parts and then the imaginary parts through an LDS array
 // of reals (half the LDS of a complex exchange, twice the barriers); the butterfly outputs
 // wait in v meanwhile, so the register peak does not grow.
@@ -668,15 +670,11 @@ void k_line_pass(PassArgs a) {
             if constexpr (C::TWM == 1) {
                 for (int q = tid; q < N; q += C::THREADS) tl[q] = tw[q];
             } else {
-                // stage-major layout: stage with span NS >= E owns entries [NS - E, NS*RAD - E)
-                for (int q = tid; q < N - E; q += C::THREADS) {
-                    int ns = E;
-                    while (q >= ns * ((N / ns < E) ? N / ns : E) - E) ns *= ((N / ns < E) ? N / ns : E);
-                    const int rad = (N / ns < E) ? N / ns : E;
-                    const int loc = q - (ns - E);
-                    const int k = loc / (rad - 1), r = loc % (rad - 1) + 1;
-                    tl[q] = tw[(r * k) * (N / (ns * rad))];
-                }
+                // the stage-major table of this E, which the plan stores after the flat one
+                // (tables for E = 4, 8, 16, 32 at N, 2N, 3N, 4N)
+                static_assert(E == 4 || E == 8 || E == 16 || E == 32, "stage-major twiddles: E = 4..32");
+                constexpr int EI = E == 4 ? 1 : (E == 8 ? 2 : (E == 16 ? 3 : 4));
+                for (int q = tid; q < N - E; q += C::THREADS) tl[q] = tw[EI * N + q];
             }
         }
         tw = tl;
