@@ -57,11 +57,11 @@ template <typename R, int N, bool STRIDED> struct PassTune {
     // grid-point steps/s on 1024^2 x 48 (8 columns with the split exchange: 7.2e10; 2 columns:
     // 7.1e10; 4 columns with both: no faster); at 512 points +2 % on 512^2 x 192, equal on x 48
     static constexpr bool C64 = !F32 && STRIDED && E == 16 && N >= 512 && N <= 1024;
-    // keep LDS twiddles only if they do not cost lines per block; FP64 rows read them
-    // stage-major (TWM 2: the flat table's radix-16 reads conflict in the LDS banks): FP64
-    // 1024-point rows 78.4 -> 75.0 us per 12-state launch, complex64 states with FP64
-    // arithmetic 144.5 -> 128.2 us per 24 states (FP32 rows: 76.3 -> 76.0 us, kept flat)
-    static constexpr int TWM = C64 ? 0 : ((TW_PREF && fit(WANT, 1) == fit(WANT, 0)) ? (R64X ? 2 : 1) : 0);
+    // keep LDS twiddles only if they do not cost lines per block, read stage-major (TWM 2:
+    // the flat table's radix-16 reads conflict in the LDS banks): FP64 1024-point rows
+    // 78.4 -> 75.0 us per 12-state launch, complex64 states with FP64 arithmetic 144.5 ->
+    // 128.2 us per 24 states; FP32 rows 76.3 -> 76.0 us, FP32 1024^2 x 96 steps +0.2 %
+    static constexpr int TWM = C64 ? 0 : ((TW_PREF && fit(WANT, 1) == fit(WANT, 0)) ? 2 : 0);
     static constexpr int LPB = C64 ? 4 : fit(WANT, TWM);
     // FP32 columns of 2048 points exchange in halves as well (86 -> 51 KB of LDS, 2 -> 3
     // blocks per CU): 1.00e11 -> 1.17e11 grid-point steps/s on 2048^2 x 24.  At 4096 points
@@ -77,7 +77,8 @@ template <typename R, int N, bool STRIDED> struct PassTune {
 // Small passes (lines up to 2048 points and few lines in the whole batch: one 256^2 state,
 // a few 128^2 states, one 1024^2 state) are bound by the latency of one block, not by HBM:
 // fewer elements per thread (E = N/128, 4..16) and 1-2 lines per block spread the lines over
-// many more CUs and shorten each thread's chain.  Twiddles come from LDS.  Measured on
+// many more CUs and shorten each thread's chain.  Twiddles come from LDS, stage-major (config
+// 1 solve 10.84 -> 10.59 ms, config 3 FP64 step 42.7 -> 42.1 us, FP32 unchanged).  Measured on
 // MI355X: one 256^2 FP64 state 6.8 -> 4.7 us per pass (the back-to-back launch floor is
 // 2.5 us).  The crossover to the batch configuration (back-to-back passes, us):
 //   1024 lines: small wins or ties everywhere (1024^2 FP32 row 7.5 vs 8.6, column 10.4 vs 11.3);
@@ -116,7 +117,7 @@ using PassK = KCfg<R, N, PassTune<R, N, STRIDED>::E, PassTune<R, N, STRIDED>::LP
                    PassTune<R, N, STRIDED>::XS, S, Q>;
 
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0, typename S = R, int Q = 0>
-using PassKS = KCfg<R, N, SmallTune<R, N, STRIDED>::E, SmallTune<R, N, STRIDED>::LPB, STRIDED, 1, DA,
+using PassKS = KCfg<R, N, SmallTune<R, N, STRIDED>::E, SmallTune<R, N, STRIDED>::LPB, STRIDED, 2, DA,
                     DB, GEN, 1, (GEN || OBS >= 3) ? 0 : SmallTune<R, N, STRIDED>::PF, 4, 5, OBS, 0, S, Q>;
 
 template <typename C>
