@@ -110,9 +110,20 @@ template <typename R, int N, bool STRIDED> struct SmallTune {
 // Column exchanges in real and imaginary halves pad one row per 32 positions (PADC 5)
 // instead of 16: +2 % on 1024^2 x 96 mixed (8: +0.7 %, 64 and 128: slower); the row pad
 // stays at 32 positions (16 is 3 % slower, 64 no faster).
+// The step passes of complex64 states with FP64 arithmetic over 1024-point rows go 4 rows to
+// a 256-thread block under the register budget of 3 waves per SIMD (152 instead of 174
+// VGPRs, no spill; 51 KB of LDS, 3 blocks per CU): 129.4 -> 122.7 us per 24-state launch.
+// FP64 states lose with it (75.4 -> 86.5 us per 12 states), and the kinds with block
+// partials keep the tiling that obs_lpb reports.
+template <typename R, int N, bool STRIDED, int GEN, int OBS, typename S> constexpr bool c64_row3() {
+    return sizeof(R) == 8 && sizeof(S) == 4 && !STRIDED && N == 1024 && GEN == 0 && OBS == 0;
+}
+
 template <typename R, int N, bool STRIDED, int DA, int DB, int GEN, int OBS = 0, typename S = R, int Q = 0>
-using PassK = KCfg<R, N, PassTune<R, N, STRIDED>::E, PassTune<R, N, STRIDED>::LPB, STRIDED,
-                   PassTune<R, N, STRIDED>::TWM, DA, DB, GEN, 1, PassTune<R, N, STRIDED>::PF,
+using PassK = KCfg<R, N, PassTune<R, N, STRIDED>::E,
+                   c64_row3<R, N, STRIDED, GEN, OBS, S>() ? 4 : PassTune<R, N, STRIDED>::LPB, STRIDED,
+                   PassTune<R, N, STRIDED>::TWM, DA, DB, GEN, c64_row3<R, N, STRIDED, GEN, OBS, S>() ? 3 : 1,
+                   PassTune<R, N, STRIDED>::PF,
                    (PassTune<R, N, STRIDED>::XS && STRIDED) ? 5 : 4, PassTune<R, N, STRIDED>::PADR, OBS,
                    PassTune<R, N, STRIDED>::XS, S, Q>;
 
