@@ -52,7 +52,7 @@ template <typename R, int N, bool STRIDED> struct PassTune {
                int(sizeof(typename Cx<R>::T));
     }
     // FP64 columns of 512 and 1024 points go 4 to a 256-thread block with the twiddles read
-    // through the cache (70 instead of 155 KB of LDS at 1024 points), so that two blocks
+    // through the cache (70 instead of 155 KB of LDS at 1024 points; TWM 3), so that two blocks
     // share a CU and one block's loads and stores overlap the other's FFTs: 6.7e10 -> 7.35e10
     // grid-point steps/s on 1024^2 x 48 (8 columns with the split exchange: 7.2e10; 2 columns:
     // 7.1e10; 4 columns with both: no faster); at 512 points +2 % on 512^2 x 192, equal on x 48
@@ -60,8 +60,11 @@ template <typename R, int N, bool STRIDED> struct PassTune {
     // keep LDS twiddles only if they do not cost lines per block, read stage-major (TWM 2:
     // the flat table's radix-16 reads conflict in the LDS banks): FP64 1024-point rows
     // 78.4 -> 75.0 us per 12-state launch, complex64 states with FP64 arithmetic 144.5 ->
-    // 128.2 us per 24 states; FP32 rows 76.3 -> 76.0 us, FP32 1024^2 x 96 steps +0.2 %
-    static constexpr int TWM = C64 ? 0 : ((TW_PREF && fit(WANT, 1) == fit(WANT, 0)) ? 2 : 0);
+    // 128.2 us per 24 states; FP32 rows 76.3 -> 76.0 us, FP32 1024^2 x 96 steps +0.2 %.
+    // The FP64 columns below read the stage-major table through the cache (TWM 3: a
+    // butterfly's 15 twiddles in two cache lines instead of 15): 84.3 -> 80.4 us per
+    // 12-state 1024-point launch, complex64 states 138 -> 136 us per 24
+    static constexpr int TWM = C64 ? 3 : ((TW_PREF && fit(WANT, 1) == fit(WANT, 0)) ? 2 : 0);
     static constexpr int LPB = C64 ? 4 : fit(WANT, TWM);
     // FP32 columns of 2048 points exchange in halves as well (86 -> 51 KB of LDS, 2 -> 3
     // blocks per CU): 1.00e11 -> 1.17e11 grid-point steps/s on 2048^2 x 24.  At 4096 points

@@ -272,7 +272,8 @@ template <bool STRIDED, int N, int LPB, int PADC = 4, int PADR = 5> struct LdsLa
 // TWM 0/1: tw is the flat table exp(-2 pi i t / N) (global or LDS).  TWM 2: tw is the
 // stage-major LDS table: for the stage of span NS and radix RAD, entry (NS - E) + k*(RAD-1)
 // + (r-1) holds exp(-2 pi i r k / (NS RAD)), so a butterfly reads its RAD-1 twiddles at
-// consecutive addresses from one base.  In the flat table the lanes of a radix-16 stage read
+// consecutive addresses from one base; TWM 3 reads that table through the cache.  In the flat
+// table the lanes of a radix-16 stage read
 // entries r k N / (NS RAD) (k = lane mod NS), which for FP64 fall into one or two LDS banks
 // (8- to 16-way conflicts); the stage-major reads of 8 lanes cover distinct banks.
 // XS 1: the exchange moves the real parts and then the imaginary parts through an LDS array
@@ -292,10 +293,10 @@ __device__ __forceinline__ void stockham(T2 *v, T2 *L, int l, int u, const T2 *_
 #pragma unroll
         for (int r = 0; r < RAD; ++r) x[r] = v[m + r * M];
         if constexpr (NS > 1) {
-            const T2 *tws = (TWM == 2) ? tw + (NS - E) + k * (RAD - 1) : tw;
+            const T2 *tws = (TWM >= 2) ? tw + (NS - E) + k * (RAD - 1) : tw;
 #pragma unroll
             for (int r = 1; r < RAD; ++r) {
-                const T2 w = (TWM == 2) ? tws[r - 1] : tw[(r * k) * (N / (NS * RAD))];
+                const T2 w = (TWM >= 2) ? tws[r - 1] : tw[(r * k) * (N / (NS * RAD))];
                 if constexpr (DIR > 0) x[r] = cmul_conj(x[r], w);
                 else x[r] = cmul(x[r], w);
             }
@@ -387,7 +388,7 @@ struct KCfg {
     static constexpr int XCH_ELEMS = XS ? (LAY::SIZE + 1) / 2 : LAY::SIZE;   // in complex units
     // OBS 3: elements whose weight gathers are in flight together (register budget)
     static constexpr int OBS3_KB = E < 4 ? E : (E >= 16 ? 1 : 4);
-    static constexpr int BASE_ELEMS = XCH_ELEMS + (TWM != 0 ? N : 0);
+    static constexpr int BASE_ELEMS = XCH_ELEMS + ((TWM == 1 || TWM == 2) ? N : 0);
     // OBS 3: the cyclic weights (kVnMaxNw rows of N complex FP64) and the closing element
     // vector (N complex) staged in LDS after the exchange and twiddle regions, where they fit
     static constexpr int OBS3_WOFF = (BASE_ELEMS + 1) & ~1;                 // 16-byte aligned
@@ -663,7 +664,11 @@ void k_line_pass(PassArgs a) {
     if constexpr ((C::Q & 2) && (STRIDED || T > 64)) {
         if (tid < LPB) q_lmax[tid] = 0u;
     }
-    if constexpr (C::TWM != 0) {
+    if constexpr (C::TWM == 3) {
+        // the stage-major table of this E read through the cache (see TWM 2 below)
+        static_assert(E == 4 || E == 8 || E == 16 || E == 32, "stage-major twiddles: E = 4..32");
+        tw += (E == 4 ? 1 : (E == 8 ? 2 : (E == 16 ? 3 : 4))) * N;
+    } else if constexpr (C::TWM != 0) {
         // twiddle table into LDS; the first exchange barrier of any FFT precedes its first use
         T2 *tl = smem + C::XCH_ELEMS;
         if ((C::DA == 2 ? a.dir_a != 0 : C::DA != 0) || (C::DB == 2 ? a.dir_b != 0 : C::DB != 0)) {
