@@ -81,7 +81,8 @@ template <typename R, int N, bool STRIDED> struct PassTune {
 // a few 128^2 states, one 1024^2 state) are bound by the latency of one block, not by HBM:
 // fewer elements per thread (E = N/128, 4..16) and 1-2 lines per block spread the lines over
 // many more CUs and shorten each thread's chain.  Twiddles come from LDS, stage-major (config
-// 1 solve 10.84 -> 10.59 ms, config 3 FP64 step 42.7 -> 42.1 us, FP32 unchanged).  Measured on
+// 1 solve 10.84 -> 10.59 ms, config 3 FP64 step 42.7 -> 42.1 us, FP32 unchanged; the same
+// table read through the cache instead: config 3 -1.8 %, 512^2 Gibbs solves +2-3 %).  Measured on
 // MI355X: one 256^2 FP64 state 6.8 -> 4.7 us per pass (the back-to-back launch floor is
 // 2.5 us).  The crossover to the batch configuration (back-to-back passes, us):
 //   1024 lines: small wins or ties everywhere (1024^2 FP32 row 7.5 vs 8.6, column 10.4 vs 11.3);
