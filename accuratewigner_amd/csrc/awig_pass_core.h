@@ -403,11 +403,11 @@ struct KCfg {
 // belong to one item):  Re sum conj(w_q) v for q < obs_nw, then (obs_norm2) sum |v|^2, with
 // real weights obs_n2w when given, into obs_partials[item][block][component].  Each thread
 // adds its elements, then the waves and then the waves of the block are added in order.
-// Row step passes of up to 4 elements per thread (the single-state latency configuration)
-// load the first kObsPre weight arrays with the data, so that their latency overlaps the FFTs
-// instead of following the second one.
+// Step passes of up to 4 elements per thread (the single-state latency configuration) load
+// the first kObsPre weight arrays (and the real weights of the norm2 component) with the data,
+// so that their latency overlaps the FFTs instead of following them.
 constexpr int kObsPre = 4;
-template <typename C> constexpr int obs_prefetch() { return (C::OBS == 1 && C::E <= 4) ? kObsPre : 0; }
+template <typename C> constexpr int obs_prefetch() { return ((C::OBS == 1 || C::OBS == 2) && C::E <= 4) ? kObsPre : 0; }
 
 // Adds each of the nc (uniform) components of acc over the block, in a fixed order (the
 // lanes of each wave, then the waves), into obs_partials[item][block][component].
@@ -441,7 +441,8 @@ __device__ __forceinline__ void store_block_partials(const double (&acc)[8], int
 template <typename C, typename T2>
 __device__ __forceinline__ void obs_block_partials(const PassArgs &a, const T2 *v, bool active, int off0,
                                                    int kstep, long long blk, int nl, int tid,
-                                                   const double2 (*wpre)[C::E] = nullptr) {
+                                                   const double2 (*wpre)[C::E] = nullptr,
+                                                   const double *n2pre = nullptr) {
     constexpr int N = C::N, E = C::E;
     constexpr int NPRE = obs_prefetch<C>();
     static_assert(C::THREADS % 64 == 0, "weighted-sum partials of whole waves");
@@ -465,7 +466,7 @@ __device__ __forceinline__ void obs_block_partials(const PassArgs &a, const T2 *
             }
             if (a.obs_norm2) {
                 double m2 = x * x + y * y;
-                if (a.obs_n2w) m2 *= a.obs_n2w[off0 + (long long)k * kstep];
+                if (a.obs_n2w) m2 *= (NPRE > 0) ? n2pre[k] : a.obs_n2w[off0 + (long long)k * kstep];
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
                     if (q == nw) acc[q] += m2;
@@ -726,6 +727,7 @@ void k_line_pass(PassArgs a) {
     }
     constexpr int NPRE = obs_prefetch<C>();
     double2 wpre[NPRE > 0 ? NPRE : 1][E];
+    double n2pre[NPRE > 0 ? E : 1];
     if constexpr (NPRE > 0) {
         if (active) {
             const long long nit = (long long)N * nl;
@@ -735,6 +737,10 @@ void k_line_pass(PassArgs a) {
 #pragma unroll
                     for (int k = 0; k < E; ++k) wpre[q][k] = a.obs_w[q * nit + off0 + (long long)k * kstep];
                 }
+            }
+            if (a.obs_norm2 && a.obs_n2w) {
+#pragma unroll
+                for (int k = 0; k < E; ++k) n2pre[k] = a.obs_n2w[off0 + (long long)k * kstep];
             }
         }
     }
@@ -824,7 +830,7 @@ void k_line_pass(PassArgs a) {
         }
     }
 
-    if constexpr (C::OBS == 2) obs_block_partials<C>(a, v, active, off0, kstep, blk, nl, tid);
+    if constexpr (C::OBS == 2) obs_block_partials<C>(a, v, active, off0, kstep, blk, nl, tid, wpre, n2pre);
     // The second FFT would reuse the first one's LDS addresses (exchange reads and writes
     // and twiddle reads: a VGPR each per element, ~80 at E = 32) and keep them live through
     // the factor multiply.  Hiding the lane's line and position from the compiler here makes
@@ -853,7 +859,7 @@ void k_line_pass(PassArgs a) {
             if (par & 1) v[k] = cscale(v[k], R(-1));
         }
     }
-    if constexpr (C::OBS == 1) obs_block_partials<C>(a, v, active, off0, kstep, blk, nl, tid, wpre);
+    if constexpr (C::OBS == 1) obs_block_partials<C>(a, v, active, off0, kstep, blk, nl, tid, wpre, n2pre);
     if constexpr (C::OBS == 3) store_block_partials<C>(cacc, 1 + a.obs_nd + a.obs_nw, a, blk, nl, tid);
     if constexpr (C::OBS == 4) store_block_partials<C>(cacc, a.obs_nw, a, blk, nl, tid);
     int q_e = 0;
