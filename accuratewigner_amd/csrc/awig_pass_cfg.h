@@ -19,9 +19,9 @@ constexpr int kLdsBudget = 160 * 1024;
 //    1024^2 x 96, neutral at 512^2).  FP64 passes keep the complex exchange: with split
 //    halves they reach 3 waves only by spilling, and measured no faster;
 //  * rows: 4 lines per 256-thread block in FP32, 2 lines per 128-thread block in FP64; FP32
-//    rows of 256 and of 1024+ points load their factor vectors with the data (PF, 4-7 %
-//    faster on MALL-resident batches; at 512 points it costs a wave per SIMD and 1.5 %; FP64
-//    rows go from 244 to 282 VGPRs and lose 40 %);
+//    rows of 256 and of 2048+ points load their factor vectors with the data (PF, 4-7 %
+//    faster on MALL-resident batches when measured; at 512 points it costs a wave per SIMD
+//    and 1.5 %, at 1024 points now 3 %; FP64 rows go from 244 to 282 VGPRs and lose 40 %);
 //  * columns: 8 adjacent columns per block (64-byte row segments in FP32, 128 in FP64; FP64
 //    columns of 512 and 1024 points: 4, see C64);
 //  * twiddles in LDS (FP32 always, FP64 columns when they fit) or read through the cache;
@@ -74,7 +74,10 @@ template <typename R, int N, bool STRIDED> struct PassTune {
     // 94.3 -> 87.6 us per 12-state 1024^2 launch (complex64 states with FP64 arithmetic:
     // 161.8 -> 160.6 us per 24 states); either change alone gains 1-3 %
     static constexpr int XS = (WIDE || (F32 && STRIDED && N == 2048) || (F32 && !STRIDED && E == 16) || R64X) ? 1 : 0;
-    static constexpr int PF = (F32 && !STRIDED && (N == 256 || N >= 1024)) ? 1 : 0;
+    // (FP32 rows of 1024 points without the prefetch: 85 VGPRs and 5 waves per SIMD, 74.5-75.5
+    // -> 72.2-73.0 us per 24-state launch, since the exchange addressing and the butterflies
+    // were slimmed; with it 108 VGPRs and 4 waves)
+    static constexpr int PF = (F32 && !STRIDED && (N == 256 || N > 1024)) ? 1 : 0;
 };
 
 // Small passes (lines up to 2048 points and few lines in the whole batch: one 256^2 state,
