@@ -63,7 +63,7 @@ def test_serving_fp64_matches_oracle():
         assert j.latency is not None and j.latency >= 0.0
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 2e-5)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 2e-5), ("c64", 2e-6)])
 def test_serving_alone_equals_in_mix(precision, tol):
     from accuratewigner_amd.serving import Server
     target, jobs = _jobs(precision)
@@ -77,6 +77,18 @@ def test_serving_alone_equals_in_mix(precision, tol):
         j2 = Job(j.kind, j.N, precision, c4=j.c4, steps=j.steps, beta=j.beta, x0=j.x0, p0=j.p0, job_id=100 + j.id)
         Server(hbm_budget_bytes=1 << 30, chunk=16, max_slots=8).run([j2], realtime=False)
         assert _rel(j.result, j2.result) < tol, (j.id, _rel(j.result, j2.result))
+
+
+def test_serving_c64_within_the_error_bound_of_fp64():
+    """c64 pools (complex64 slots, FP64 arithmetic, line fixed point inside each chunk's step
+    sequence): every job of the mix within 1e-6 of the same job served alone in FP64."""
+    from accuratewigner_amd.serving import Job, Server
+    _, jobs = _jobs("c64")
+    Server(hbm_budget_bytes=1 << 30, chunk=16, max_slots=8).run(jobs, realtime=False)
+    for j in jobs:
+        ref = Job(j.kind, j.N, "fp64", c4=j.c4, steps=j.steps, beta=j.beta, x0=j.x0, p0=j.p0, job_id=200 + j.id)
+        Server(hbm_budget_bytes=1 << 30, chunk=16, max_slots=8).run([ref], realtime=False)
+        assert _rel(j.result, ref.result) < 1e-6, (j.id, _rel(j.result, ref.result))
 
 
 def test_serving_admission_waits_for_free_slots():
